@@ -1,0 +1,500 @@
+// Host side of the 1D complex engine: plans (radix factorisation, FP64
+// twiddle / chirp tables resident in HBM, four-step and Bluestein sub-plans)
+// and the launchers that run a plan over a set of lines.
+//
+// Plan kinds, mirroring the reference's three plans (DoubleFFT_1D.java:56-60,
+// 117-188) but chosen for the GPU:
+//   TPL   n a power of two in [16, 4096]: compile-time stockham_kernel
+//   GEN   n <= LDS limit with factors in {2,3,5,7,11,13}: generic_kernel
+//   FOUR  smooth n above the LDS limit: n = n1*n2, two LDS passes
+//   BLUE  any other n: Bluestein chirp-z over a smooth length m >= 2n-1
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "jt_generic.hpp"
+#include "jt_io.hpp"
+#include "jt_stockham.hpp"
+
+namespace jt {
+
+// ----------------------------------------------------------------------------
+// errors
+// ----------------------------------------------------------------------------
+struct JtError : std::runtime_error {
+    int code;
+    JtError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+enum { ST_OK = 0, ST_BAD_SIZE = 1, ST_NOT_POW2 = 2, ST_TOO_BIG = 3, ST_HIP = 4, ST_OOM = 5,
+       ST_BAD_ARG = 6, ST_UNSUPPORTED = 7 };
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory) throw JtError(ST_OOM, std::string(what) + ": out of device memory");
+        throw JtError(ST_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+#define JT_HIP(x) ::jt::hip_check((x), #x)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t b) {
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+        bytes = b;
+        if (b) JT_HIP(hipMalloc(&p, b));
+    }
+    template <typename U>
+    U* as() const { return static_cast<U*>(p); }
+};
+
+template <typename T>
+inline void upload(DevBuf& d, const std::vector<cpx<T>>& h) {
+    d.alloc(h.size() * sizeof(cpx<T>));
+    if (!h.empty()) JT_HIP(hipMemcpy(d.p, h.data(), d.bytes, hipMemcpyHostToDevice));
+}
+
+// exp(-2 pi i num/den) in long double, num reduced exactly
+template <typename T>
+inline cpx<T> expm(int64_t num, int64_t den) {
+    num %= den;
+    if (num < 0) num += den;
+    const long double PI2 = 6.283185307179586476925286766559005768L;
+    long double a = PI2 * (long double)num / (long double)den;
+    return cpx<T>{(T)cosl(a), (T)-sinl(a)};
+}
+
+inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// ----------------------------------------------------------------------------
+// factorisation
+// ----------------------------------------------------------------------------
+inline bool factor_radices(int64_t n, std::vector<int>& out) {
+    std::vector<int> r;
+    int64_t m = n;
+    int e2 = 0;
+    while (m % 2 == 0) {
+        m /= 2;
+        e2++;
+    }
+    while (e2 >= 4) {
+        r.push_back(16);
+        e2 -= 4;
+    }
+    if (e2 == 3) r.push_back(8);
+    else if (e2 == 2) r.push_back(4);
+    else if (e2 == 1) r.push_back(2);
+    for (int p : {3, 5, 7, 11, 13})
+        while (m % p == 0) {
+            r.push_back(p);
+            m /= p;
+        }
+    if (m != 1) return false;
+    out = r;
+    return true;
+}
+
+// radix lists of the compile-time kernels (must match dispatch_tpl)
+inline bool tpl_radices(int64_t n, std::vector<int>& r) {
+    switch (n) {
+        case 4096: r = {16, 16, 16}; return true;
+        case 2048: r = {16, 16, 8}; return true;
+        case 1024: r = {16, 16, 4}; return true;
+        case 512: r = {8, 8, 8}; return true;
+        case 256: r = {16, 16}; return true;
+        case 128: r = {16, 8}; return true;
+        case 64: r = {8, 8}; return true;
+        case 32: r = {8, 4}; return true;
+        case 16: r = {16}; return true;
+        default: return false;
+    }
+}
+
+template <typename T>
+inline int64_t gen_max() { return sizeof(T) == 8 ? 4096 : 8192; }
+
+template <typename T>
+std::vector<cpx<T>> stage_twiddles(const std::vector<int>& radices, std::vector<int>* offs) {
+    std::vector<cpx<T>> tw;
+    int64_t ns = 1;
+    for (int R : radices) {
+        if (offs) offs->push_back((int)tw.size());
+        for (int64_t jm = 0; jm < ns; jm++)
+            for (int r = 1; r < R; r++) tw.push_back(expm<T>(jm * r, ns * R));
+        ns *= R;
+    }
+    if (tw.empty()) tw.push_back(cpx<T>{T(1), T(0)});
+    return tw;
+}
+
+// Host long-double DFT by recursive mixed radix (plan time only: Bluestein
+// kernel spectrum).  sign -1 forward.
+inline void host_fft_ld(std::vector<long double>& re, std::vector<long double>& im, int sign) {
+    const size_t n = re.size();
+    if (n <= 1) return;
+    size_t p = 0;
+    for (size_t q : {2, 3, 5, 7, 11, 13})
+        if (n % q == 0) {
+            p = q;
+            break;
+        }
+    if (!p) p = n;  // direct DFT
+    const long double PI2 = 6.283185307179586476925286766559005768L;
+    if (p == n) {
+        std::vector<long double> or_(n), oi(n);
+        for (size_t k = 0; k < n; k++) {
+            long double sr = 0, si = 0;
+            for (size_t j = 0; j < n; j++) {
+                long double a = sign * PI2 * (long double)((j * k) % n) / (long double)n;
+                sr += re[j] * cosl(a) - im[j] * sinl(a);
+                si += re[j] * sinl(a) + im[j] * cosl(a);
+            }
+            or_[k] = sr;
+            oi[k] = si;
+        }
+        re = or_;
+        im = oi;
+        return;
+    }
+    const size_t m = n / p;
+    std::vector<std::vector<long double>> sr(p, std::vector<long double>(m)), si(p, std::vector<long double>(m));
+    for (size_t q = 0; q < p; q++)
+        for (size_t j = 0; j < m; j++) {
+            sr[q][j] = re[j * p + q];
+            si[q][j] = im[j * p + q];
+        }
+    for (size_t q = 0; q < p; q++) host_fft_ld(sr[q], si[q], sign);
+    for (size_t k = 0; k < n; k++) {
+        long double accr = 0, acci = 0;
+        for (size_t q = 0; q < p; q++) {
+            long double a = sign * PI2 * (long double)((q * k) % n) / (long double)n;
+            long double c = cosl(a), s = sinl(a);
+            long double xr = sr[q][k % m], xi = si[q][k % m];
+            accr += xr * c - xi * s;
+            acci += xr * s + xi * c;
+        }
+        re[k] = accr;
+        im[k] = acci;
+    }
+}
+
+inline int64_t next_smooth(int64_t x) {
+    // smallest 2^a 3^b 5^c >= x
+    int64_t best = INT64_MAX;
+    for (int64_t p2 = 1; p2 < 4 * x; p2 *= 2)
+        for (int64_t p3 = p2; p3 < 4 * x; p3 *= 3)
+            for (int64_t p5 = p3; p5 < 4 * x; p5 *= 5)
+                if (p5 >= x && p5 < best) best = p5;
+    return best;
+}
+
+// ----------------------------------------------------------------------------
+// the plan
+// ----------------------------------------------------------------------------
+enum PlanKind { K_TPL = 0, K_GEN = 1, K_FOUR = 2, K_BLUE = 3 };
+
+template <typename T>
+struct Fft1D {
+    int64_t n = 0;
+    int kind = K_GEN;
+    std::vector<int> radices;
+    GenPlan gp{};
+    DevBuf tw;
+    // four-step
+    int64_t n1 = 0, n2 = 0;
+    std::unique_ptr<Fft1D<T>> p1, p2;
+    DevBuf hi, lo;
+    // Bluestein
+    int64_t m = 0;
+    std::unique_ptr<Fft1D<T>> pm;
+    DevBuf chirp, bspec;
+
+    bool single() const { return kind == K_TPL || kind == K_GEN; }
+    // workspace (complex elements) for `lines` contiguous lines
+    int64_t workspace(int64_t lines) const {
+        if (kind == K_FOUR) return lines * n;
+        if (kind == K_BLUE) return lines * m + pm->workspace(lines);
+        return 0;
+    }
+};
+
+template <typename T>
+std::unique_ptr<Fft1D<T>> make_fft1d(int64_t n) {
+    auto p = std::make_unique<Fft1D<T>>();
+    p->n = n;
+    std::vector<int> r;
+    bool tpl = tpl_radices(n, r);
+    if (tpl || (n <= gen_max<T>() && factor_radices(n, r))) {
+        p->kind = tpl ? K_TPL : K_GEN;
+        p->radices = r;
+        std::vector<int> offs;
+        upload(p->tw, stage_twiddles<T>(r, &offs));
+        p->gp.n = (int)n;
+        p->gp.nstages = (int)r.size();
+        for (size_t i = 0; i < r.size(); i++) {
+            p->gp.radix[i] = r[i];
+            p->gp.twoff[i] = offs[i];
+        }
+        return p;
+    }
+    if (factor_radices(n, r)) {
+        // four-step: n1 (column FFT) * n2 (row FFT), both single-kernel
+        int64_t best1 = 0;
+        for (int64_t a = 2; a <= gen_max<T>(); a++) {
+            if (n % a) continue;
+            int64_t b = n / a;
+            if (b > gen_max<T>()) continue;
+            std::vector<int> ra, rb;
+            if (!factor_radices(a, ra) || !factor_radices(b, rb)) continue;
+            // prefer balanced split, n1 a power of two when possible
+            if (!best1 || std::llabs(a - b) < std::llabs(best1 - n / best1)) best1 = a;
+        }
+        if (best1) {
+            p->kind = K_FOUR;
+            p->n1 = best1;
+            p->n2 = n / best1;
+            p->p1 = make_fft1d<T>(p->n1);
+            p->p2 = make_fft1d<T>(p->n2);
+            std::vector<cpx<T>> hi((n >> 8) + 1), lo(256);
+            for (int64_t i = 0; i < (int64_t)hi.size(); i++) hi[i] = expm<T>(i << 8, n);
+            for (int64_t i = 0; i < 256; i++) lo[i] = expm<T>(i, n);
+            upload(p->hi, hi);
+            upload(p->lo, lo);
+            return p;
+        }
+    }
+    // Bluestein (DoubleFFT_1D.java:1864-2107): chirp c_j = exp(i pi (j^2 mod 2n)/n)
+    p->kind = K_BLUE;
+    p->m = next_smooth(2 * n - 1);
+    p->pm = make_fft1d<T>(p->m);
+    std::vector<cpx<T>> ch(n);
+    std::vector<long double> br(p->m, 0.0L), bi(p->m, 0.0L);
+    const long double PI = 3.141592653589793238462643383279502884L;
+    int64_t k = 0;
+    for (int64_t j = 0; j < n; j++) {
+        if (j) {
+            k += 2 * j - 1;
+            if (k >= 2 * n) k -= 2 * n;
+        }
+        long double a = PI * (long double)k / (long double)n;
+        long double c = cosl(a), s = sinl(a);
+        ch[j] = cpx<T>{(T)c, (T)s};
+        br[j] = c;
+        bi[j] = s;
+        if (j) {
+            br[p->m - j] = c;
+            bi[p->m - j] = s;
+        }
+    }
+    host_fft_ld(br, bi, -1);
+    std::vector<cpx<T>> bs(p->m);
+    for (int64_t i = 0; i < p->m; i++) bs[i] = cpx<T>{(T)(br[i] / p->m), (T)(bi[i] / p->m)};
+    upload(p->chirp, ch);
+    upload(p->bspec, bs);
+    return p;
+}
+
+// ----------------------------------------------------------------------------
+// launch: compile-time kernels
+// ----------------------------------------------------------------------------
+template <typename T, int N, int TL, bool COL>
+struct TplShape {
+    static constexpr int lines_row = (256 / TL) > 0 ? (256 / TL) : 1;
+    // column mode: enough adjacent lines for >= 64-byte segments, <= 512 threads
+    static constexpr int want_col = (64 / (int)sizeof(cpx<T>)) > 0 ? 64 / (int)sizeof(cpx<T>) : 1;
+    static constexpr int col_threads_cap = 512 / TL > 0 ? 512 / TL : 1;
+    static constexpr int w_col0 = want_col > col_threads_cap ? col_threads_cap : want_col;
+    static constexpr int w_col = (w_col0 * TL < 256) ? (256 / TL) : w_col0;
+    static constexpr int W = COL ? w_col : lines_row;
+    static constexpr int bytes_full = 2 * LdsGeom<N, W>::size * (int)sizeof(T);
+    static constexpr bool split = bytes_full > 65536;
+};
+
+template <typename T, int DIR, int N, int TL, bool COL, class IO, int... Rs>
+void go_tpl(IO io, const cpx<T>* tw, hipStream_t st) {
+    using S = TplShape<T, N, TL, COL>;
+    constexpr int W = S::W;
+    constexpr bool SPLIT = S::split && !IO::pre_lds && !IO::post_lds;
+    int64_t groups = (io.nlines + W - 1) / W;
+    if (groups <= 0) return;
+    hipLaunchKernelGGL((stockham_kernel<T, DIR, N, TL, W, COL, SPLIT, IO, Rs...>), dim3((unsigned)groups),
+                       dim3(TL * W), 0, st, io, tw);
+    JT_HIP(hipGetLastError());
+}
+
+template <typename T, int DIR, bool COL, class IO>
+bool dispatch_tpl(int64_t n, IO io, const cpx<T>* tw, hipStream_t st) {
+    switch (n) {
+        case 4096: go_tpl<T, DIR, 4096, 256, COL, IO, 16, 16, 16>(io, tw, st); return true;
+        case 2048: go_tpl<T, DIR, 2048, 128, COL, IO, 16, 16, 8>(io, tw, st); return true;
+        case 1024: go_tpl<T, DIR, 1024, 64, COL, IO, 16, 16, 4>(io, tw, st); return true;
+        case 512: go_tpl<T, DIR, 512, 64, COL, IO, 8, 8, 8>(io, tw, st); return true;
+        case 256: go_tpl<T, DIR, 256, 16, COL, IO, 16, 16>(io, tw, st); return true;
+        case 128: go_tpl<T, DIR, 128, 8, COL, IO, 16, 8>(io, tw, st); return true;
+        case 64: go_tpl<T, DIR, 64, 8, COL, IO, 8, 8>(io, tw, st); return true;
+        case 32: go_tpl<T, DIR, 32, 4, COL, IO, 8, 4>(io, tw, st); return true;
+        case 16: go_tpl<T, DIR, 16, 1, COL, IO, 16>(io, tw, st); return true;
+        default: return false;
+    }
+}
+
+template <typename T, int DIR, class IO>
+void go_gen(const Fft1D<T>& p, IO io, hipStream_t st) {
+    if (io.nlines <= 0) return;
+    int n = (int)p.n;
+    int np = n + (n >> 5) + 1;
+    size_t shm = 4 * (size_t)np * sizeof(T);
+    int nt = n / 4;
+    nt = nt < 64 ? 64 : (nt > 256 ? 256 : ((nt + 63) / 64) * 64);
+    static bool attr_set = false;  // per instantiation
+    if (!attr_set) {
+        JT_HIP(hipFuncSetAttribute((const void*)generic_kernel<T, DIR, IO>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_set = true;
+    }
+    int64_t lines = io.nlines;
+    unsigned gx = (unsigned)(lines < 65535 ? lines : 65535);
+    unsigned gy = (unsigned)((lines + gx - 1) / gx);
+    hipLaunchKernelGGL((generic_kernel<T, DIR, IO>), dim3(gx, gy), dim3(nt), shm, st, io,
+                       p.tw.template as<cpx<T>>(), p.gp);
+    JT_HIP(hipGetLastError());
+}
+
+// Run a single-kernel plan over the lines described by io.
+template <typename T, int DIR, bool COL, class IO>
+void run_single(const Fft1D<T>& p, IO io, hipStream_t st) {
+    if (p.kind == K_TPL) {
+        if (dispatch_tpl<T, DIR, COL>(p.n, io, p.tw.template as<cpx<T>>(), st)) return;
+    }
+    if (p.kind == K_GEN || p.kind == K_TPL) {
+        // generic kernel needs a radix plan
+        go_gen<T, DIR>(p, io, st);
+        return;
+    }
+    throw JtError(ST_UNSUPPORTED, "run_single on a multi-pass plan");
+}
+
+// ----------------------------------------------------------------------------
+// elementwise kernels (Bluestein, real-transform helpers)
+// ----------------------------------------------------------------------------
+template <typename T, int DIR>
+__global__ void blue_premul(const cpx<T>* __restrict__ x, cpx<T>* __restrict__ y,
+                            const cpx<T>* __restrict__ ch, int64_t n, int64_t m, int64_t lines) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t tot = lines * m;
+    for (; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t b = i / m, j = i % m;
+        cpx<T> v{T(0), T(0)};
+        if (j < n) {
+            cpx<T> a = x[b * n + j];
+            v = DIR < 0 ? cmulc(a, ch[j]) : cmul(a, ch[j]);
+        }
+        y[i] = v;
+    }
+}
+
+template <typename T, int DIR>
+__global__ void blue_pointwise(cpx<T>* __restrict__ y, const cpx<T>* __restrict__ bs, int64_t m,
+                               int64_t lines) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t tot = lines * m;
+    for (; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t j = i % m;
+        y[i] = DIR < 0 ? cmul(y[i], bs[j]) : cmulc(y[i], bs[j]);
+    }
+}
+
+template <typename T, int DIR>
+__global__ void blue_postmul(const cpx<T>* __restrict__ y, cpx<T>* __restrict__ x,
+                             const cpx<T>* __restrict__ ch, int64_t n, int64_t m, int64_t lines,
+                             T scale) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t tot = lines * n;
+    for (; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t b = i / n, j = i % n;
+        cpx<T> v = y[b * m + j];
+        v = DIR < 0 ? cmulc(v, ch[j]) : cmul(v, ch[j]);
+        x[i] = cpx<T>{v.x * scale, v.y * scale};
+    }
+}
+
+template <typename T>
+__global__ void scale_kernel(cpx<T>* __restrict__ x, int64_t count, T s) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    for (; i < count; i += (int64_t)gridDim.x * blockDim.x) x[i] = cpx<T>{x[i].x * s, x[i].y * s};
+}
+
+inline dim3 ew_grid(int64_t count) {
+    int64_t b = (count + 255) / 256;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
+}
+
+// ----------------------------------------------------------------------------
+// run a plan over `lines` lines; complex data of line b at b*n ("contiguous")
+// or through a LineMap for single-kernel plans.
+// ----------------------------------------------------------------------------
+template <typename T, int DIR>
+void c2c_contig(const Fft1D<T>& p, const cpx<T>* in, cpx<T>* out, int64_t lines, T scale,
+                cpx<T>* ws, hipStream_t st);
+
+template <typename T, int DIR>
+void c2c_mapped(const Fft1D<T>& p, const cpx<T>* in, cpx<T>* out, LineMap lm, T scale,
+                hipStream_t st) {
+    IOStrided<T> io{in, out, lm, lm.nlines, scale, lm.es != 1};
+    if (lm.es == 1) run_single<T, DIR, false>(p, io, st);
+    else run_single<T, DIR, true>(p, io, st);
+}
+
+template <typename T, int DIR>
+void c2c_contig(const Fft1D<T>& p, const cpx<T>* in, cpx<T>* out, int64_t lines, T scale,
+                cpx<T>* ws, hipStream_t st) {
+    if (lines <= 0) return;
+    if (p.single()) {
+        LineMap lm{lines, lines, p.n, 0, 1};
+        c2c_mapped<T, DIR>(p, in, out, lm, scale, st);
+        return;
+    }
+    if (p.kind == K_FOUR) {
+        if (!p.p1->single() || !p.p2->single())
+            throw JtError(ST_UNSUPPORTED, "four-step sub-plan must be single-kernel");
+        IOFourA<T, DIR> a{in, ws, p.hi.template as<cpx<T>>(), p.lo.template as<cpx<T>>(), lines * p.n2,
+                          p.n, p.n1, p.n2, true};
+        run_single<T, DIR, true>(*p.p1, a, st);
+        IOFourB<T> b{ws, out, lines * p.n1, p.n, p.n1, p.n2, scale, false};
+        run_single<T, DIR, false>(*p.p2, b, st);
+        return;
+    }
+    // Bluestein
+    cpx<T>* y = ws;
+    cpx<T>* ws2 = ws + lines * p.m;
+    const cpx<T>* ch = p.chirp.template as<cpx<T>>();
+    hipLaunchKernelGGL((blue_premul<T, DIR>), ew_grid(lines * p.m), dim3(256), 0, st, in, y, ch, p.n, p.m,
+                       lines);
+    JT_HIP(hipGetLastError());
+    c2c_contig<T, -1>(*p.pm, y, y, lines, T(1), ws2, st);
+    hipLaunchKernelGGL((blue_pointwise<T, DIR>), ew_grid(lines * p.m), dim3(256), 0, st, y,
+                       p.bspec.template as<cpx<T>>(), p.m, lines);
+    JT_HIP(hipGetLastError());
+    c2c_contig<T, 1>(*p.pm, y, y, lines, T(1), ws2, st);
+    hipLaunchKernelGGL((blue_postmul<T, DIR>), ew_grid(lines * p.n), dim3(256), 0, st, y, out, ch, p.n,
+                       p.m, lines, scale);
+    JT_HIP(hipGetLastError());
+}
+
+}  // namespace jt

@@ -1,0 +1,146 @@
+// Runtime-radix Stockham kernel: one workgroup per line, the line held in two
+// LDS buffers (ping-pong), radices read from the kernel arguments.  Serves all
+// lengths that have no compile-time kernel: the mixed-radix sizes of the
+// reference's FFTPACK path (DoubleFFT_1D.java:6630-8008) and the sub-lengths
+// of the four-step and Bluestein plans.
+#pragma once
+#include "jt_dft.hpp"
+
+namespace jt {
+
+constexpr int GEN_MAX_STAGES = 20;
+
+struct GenPlan {
+    int n;
+    int nstages;
+    int radix[GEN_MAX_STAGES];
+    int twoff[GEN_MAX_STAGES];
+};
+
+__device__ __forceinline__ int gpos(int k) { return k + (k >> 5); }
+
+// Prime radix P (11, 13): direct O(P^2) evaluation; exp(DIR 2 pi i m/P) for
+// m in [0,P) is evaluated once per butterfly.
+template <int P, int DIR, typename T>
+__device__ __forceinline__ void dft_prime(cpx<T>* v) {
+    cpx<T> w[P];
+#pragma unroll
+    for (int m = 0; m < P; m++) {
+        T s, c;
+        if constexpr (sizeof(T) == 8) sincospi(T(2 * m) / T(P), &s, &c);
+        else sincospif(T(2 * m) / T(P), &s, &c);
+        w[m] = {c, T(DIR) * s};
+    }
+    cpx<T> out[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+        cpx<T> acc = v[0];
+#pragma unroll
+        for (int j = 1; j < P; j++) acc = cadd(acc, cmul(v[j], w[(j * k) % P]));
+        out[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < P; k++) v[k] = out[k];
+}
+
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dft_any(cpx<T>* v) {
+    if constexpr (R == 11 || R == 13) dft_prime<R, DIR>(v);
+    else dft<R, DIR>(v);
+}
+
+// One Stockham stage of compile-time radix R over LDS buffers a -> b.
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void gen_stage(const T* are, const T* aim, T* bre, T* bim,
+                                          const cpx<T>* __restrict__ tws, int n, int ns, int t,
+                                          int nt) {
+    const int m = n / R;
+    for (int j = t; j < m; j += nt) {
+        const int jm = j % ns;
+        cpx<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            int k = gpos(j + r * m);
+            v[r] = {are[k], aim[k]};
+        }
+        if (ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) v[r] = cmul(v[r], tw_dir<DIR>(tws[jm * (R - 1) + r - 1]));
+        }
+        dft_any<R, DIR>(v);
+        const int d = (j / ns) * ns * R + jm;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            int k = gpos(d + r * ns);
+            bre[k] = v[r].x;
+            bim[k] = v[r].y;
+        }
+    }
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void gen_stage_rt(int R, const T* are, const T* aim, T* bre, T* bim,
+                                             const cpx<T>* __restrict__ tws, int n, int ns, int t,
+                                             int nt) {
+    switch (R) {
+        case 2: gen_stage<2, DIR>(are, aim, bre, bim, tws, n, ns, t, nt); break;
+        case 3: gen_stage<3, DIR>(are, aim, bre, bim, tws, n, ns, t, nt); break;
+        case 4: gen_stage<4, DIR>(are, aim, bre, bim, tws, n, ns, t, nt); break;
+        case 5: gen_stage<5, DIR>(are, aim, bre, bim, tws, n, ns, t, nt); break;
+        case 7: gen_stage<7, DIR>(are, aim, bre, bim, tws, n, ns, t, nt); break;
+        case 8: gen_stage<8, DIR>(are, aim, bre, bim, tws, n, ns, t, nt); break;
+        case 11: gen_stage<11, DIR>(are, aim, bre, bim, tws, n, ns, t, nt); break;
+        case 13: gen_stage<13, DIR>(are, aim, bre, bim, tws, n, ns, t, nt); break;
+        case 16: gen_stage<16, DIR>(are, aim, bre, bim, tws, n, ns, t, nt); break;
+        default: break;
+    }
+}
+
+// IO contract as for stockham_kernel, with W == 1 and pre/post hooks working
+// on one line: pre(line, base, re, im, t, nthreads) / post(line, base, re, im,
+// t, nthreads) address LDS through gpos(k).
+template <typename T, int DIR, class IO>
+__global__ void __launch_bounds__(256) generic_kernel(IO io, const cpx<T>* __restrict__ tw,
+                                                     GenPlan plan) {
+    extern __shared__ unsigned char smem_raw[];
+    T* sm = reinterpret_cast<T*>(smem_raw);
+    const int n = plan.n;
+    const int np = n + (n >> 5) + 1;
+    T* are = sm;
+    T* aim = sm + np;
+    T* bre = sm + 2 * np;
+    T* bim = sm + 3 * np;
+    const int t = threadIdx.x;
+    const int nt = blockDim.x;
+    const int64_t line = blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
+    if (line >= io.nlines) return;  // whole workgroup exits together
+    const int64_t base = io.base(line);
+
+    if constexpr (IO::pre_lds) {
+        io.pre(line, base, are, aim, t, nt);
+    } else {
+        for (int k = t; k < n; k += nt) {
+            cpx<T> x = io.load(base, k);
+            are[gpos(k)] = x.x;
+            aim[gpos(k)] = x.y;
+        }
+    }
+    int ns = 1;
+    for (int s = 0; s < plan.nstages; s++) {
+        __syncthreads();
+        const int R = plan.radix[s];
+        gen_stage_rt<DIR>(R, are, aim, bre, bim, tw + plan.twoff[s], n, ns, t, nt);
+        ns *= R;
+        T* x;
+        x = are; are = bre; bre = x;
+        x = aim; aim = bim; bim = x;
+    }
+    __syncthreads();
+    if constexpr (IO::post_lds) {
+        io.post(line, base, are, aim, t, nt);
+    } else {
+        for (int k = t; k < n; k += nt) io.store(base, k, cpx<T>{are[gpos(k)], aim[gpos(k)]});
+    }
+}
+
+}  // namespace jt

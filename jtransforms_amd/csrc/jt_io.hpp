@@ -1,0 +1,191 @@
+// IO policies of the Stockham kernels: where a line's points come from, where
+// the results go, and the O(N) pre/post passes fused around the FFT.
+#pragma once
+#include "jt_dft.hpp"
+
+namespace jt {
+
+__device__ __forceinline__ int lpos(int k) { return k + (k >> 5); }
+
+// Lines l in [0, nlines) start at (l % n0)*s0 + (l / n0)*s1 (complex units);
+// element k of a line sits es further per step.
+struct LineMap {
+    int64_t nlines, n0, s0, s1, es;
+    __device__ __forceinline__ int64_t base(int64_t l) const { return (l % n0) * s0 + (l / n0) * s1; }
+};
+
+// Plain complex lines, optionally scaled on store (complexInverse(scale)).
+template <typename T>
+struct IOStrided {
+    static constexpr bool pre_lds = false, post_lds = false;
+    const cpx<T>* in;
+    cpx<T>* out;
+    LineMap m;
+    int64_t nlines;
+    T scale;
+    bool xcd;
+    __device__ __forceinline__ int64_t base(int64_t l) const { return m.base(l); }
+    __device__ __forceinline__ cpx<T> load(int64_t b, int k) const { return in[b + k * m.es]; }
+    __device__ __forceinline__ void store(int64_t b, int k, cpx<T> v) const {
+        out[b + k * m.es] = cpx<T>{v.x * scale, v.y * scale};
+    }
+    __device__ void pre(int64_t, int64_t, T*, T*, int, int) const {}
+    __device__ void post(int64_t, int64_t, const T*, const T*, int, int) const {}
+};
+
+// exp(DIR 2 pi i e / n) from two tables: hi[e >> 8] * lo[e & 255], both
+// holding forward values exp(-2 pi i x/n).
+template <int DIR, typename T>
+__device__ __forceinline__ cpx<T> tw2(const cpx<T>* __restrict__ hi, const cpx<T>* __restrict__ lo,
+                                     int64_t e) {
+    cpx<T> w = cmul(hi[e >> 8], lo[e & 255]);
+    return tw_dir<DIR>(w);
+}
+
+// Four-step pass A: transform length n = n1*n2 viewed as an n1 x n2 row-major
+// matrix.  Line (b, j2) is column j2 of batch b (n1 points at stride n2); the
+// result is multiplied by exp(DIR 2 pi i j2 k1 / n) and written to the same
+// position of `out`.
+template <typename T, int DIR>
+struct IOFourA {
+    static constexpr bool pre_lds = false, post_lds = false;
+    const cpx<T>* in;
+    cpx<T>* out;
+    const cpx<T>* hi;
+    const cpx<T>* lo;
+    int64_t nlines, n, n1, n2;
+    bool xcd;
+    __device__ __forceinline__ int64_t base(int64_t l) const { return (l / n2) * n + (l % n2); }
+    __device__ __forceinline__ cpx<T> load(int64_t b, int k) const { return in[b + (int64_t)k * n2]; }
+    __device__ __forceinline__ void store(int64_t b, int k, cpx<T> v) const {
+        int64_t j2 = b % n;  // column index inside the batch entry
+        int64_t e = (j2 * (int64_t)k) % n;
+        out[b + (int64_t)k * n2] = cmul(v, tw2<DIR>(hi, lo, e));
+    }
+    __device__ void pre(int64_t, int64_t, T*, T*, int, int) const {}
+    __device__ void post(int64_t, int64_t, const T*, const T*, int, int) const {}
+};
+
+// Four-step pass B: line (b, k1) is row k1 (n2 contiguous points); output
+// point k2 is X[k1 + n1*k2], written with stride n1.
+template <typename T>
+struct IOFourB {
+    static constexpr bool pre_lds = false, post_lds = false;
+    const cpx<T>* in;
+    cpx<T>* out;
+    int64_t nlines, n, n1, n2;
+    T scale;
+    bool xcd;
+    // base encodes (b, k1) as b*n + k1*n2 (the input row start)
+    __device__ __forceinline__ int64_t base(int64_t l) const { return (l / n1) * n + (l % n1) * n2; }
+    __device__ __forceinline__ cpx<T> load(int64_t b, int k) const { return in[b + k]; }
+    __device__ __forceinline__ void store(int64_t b, int k, cpx<T> v) const {
+        int64_t bb = b / n, k1 = (b % n) / n2;
+        out[bb * n + k1 + (int64_t)k * n1] = cpx<T>{v.x * scale, v.y * scale};
+    }
+    __device__ void pre(int64_t, int64_t, T*, T*, int, int) const {}
+    __device__ void post(int64_t, int64_t, const T*, const T*, int, int) const {}
+};
+
+// realForward post pass (reference: CommonUtils.rftfsub :5750-5776 and the
+// fold at DoubleFFT_1D.java:543-545).  The line holds the N-point complex FFT
+// Z of the 2N reals taken pairwise; the packed result is
+//   a[0] = X_0, a[1] = X_N, a[2k], a[2k+1] = Re, Im X_k  (0 < k < N),
+// with X_k = (Z_k + conj Z_{N-k})/2 - i w^k (Z_k - conj Z_{N-k})/2,
+// w = exp(-2 pi i/2N).  rtw[k] = w^k for k in [0, N/2].
+template <typename T>
+struct IOR2C {
+    static constexpr bool pre_lds = false, post_lds = true;
+    const cpx<T>* in;  // reals viewed as complex pairs
+    T* out;            // packed reals
+    const cpx<T>* rtw;
+    LineMap m;  // in complex units; es must be 1
+    int64_t nlines;
+    int N;
+    T scale;
+    bool xcd;
+    __device__ __forceinline__ int64_t base(int64_t l) const { return m.base(l); }
+    __device__ __forceinline__ cpx<T> load(int64_t b, int k) const { return in[b + k]; }
+    __device__ void store(int64_t, int, cpx<T>) const {}
+    __device__ void pre(int64_t, int64_t, T*, T*, int, int) const {}
+    __device__ void post(int64_t, int64_t b, const T* re, const T* im, int t, int nt) const {
+        T* o = out + 2 * b;
+        const T h = T(0.5) * scale;
+        for (int k = t; k <= N / 2; k += nt) {
+            if (k == 0) {
+                T zr = re[lpos(0)], zi = im[lpos(0)];
+                o[0] = (zr + zi) * scale;
+                o[1] = (zr - zi) * scale;
+                continue;
+            }
+            int kk = N - k;
+            cpx<T> A = {re[lpos(k)], im[lpos(k)]};
+            cpx<T> B = {re[lpos(kk)], -im[lpos(kk)]};  // conj Z_{N-k}
+            cpx<T> fe = cadd(A, B);
+            cpx<T> d = csub(A, B);
+            cpx<T> fo = {d.y, -d.x};  // -i * d
+            cpx<T> wf = cmul(rtw[k], fo);
+            // X_k = (fe + w^k fo)/2 ; X_{N-k} = conj(fe - w^k fo)/2
+            o[2 * k] = (fe.x + wf.x) * h;
+            o[2 * k + 1] = (fe.y + wf.y) * h;
+            if (kk != k) {
+                o[2 * kk] = (fe.x - wf.x) * h;
+                o[2 * kk + 1] = -(fe.y - wf.y) * h;
+            }
+        }
+    }
+};
+
+// realInverse pre pass (reference: CommonUtils.rftbsub :5806-5831 and
+// DoubleFFT_1D.java:946-989).  From the packed half spectrum X of 2N reals it
+// builds Z_k = E_k + i O_k with E_k = X_k + conj X_{N-k} and
+// O_k = (X_k - conj X_{N-k}) exp(+2 pi i k/2N); the N-point backward FFT of Z
+// is sum-normalised (x_{2m} + i x_{2m+1}) * 2N, stored times `scale`.
+template <typename T>
+struct IOC2R {
+    static constexpr bool pre_lds = true, post_lds = false;
+    const T* in;     // packed reals
+    cpx<T>* out;     // reals viewed as complex pairs
+    const cpx<T>* rtw;
+    LineMap m;
+    int64_t nlines;
+    int N;
+    T scale;
+    bool xcd;
+    __device__ __forceinline__ int64_t base(int64_t l) const { return m.base(l); }
+    __device__ cpx<T> load(int64_t, int) const { return cpx<T>{T(0), T(0)}; }
+    __device__ __forceinline__ void store(int64_t b, int k, cpx<T> v) const {
+        out[b + k] = cpx<T>{v.x * scale, v.y * scale};
+    }
+    __device__ void pre(int64_t, int64_t b, T* re, T* im, int t, int nt) const {
+        const T* a = in + 2 * b;
+        for (int k = t; k <= N / 2; k += nt) {
+            if (k == 0) {
+                T x0 = a[0], xn = a[1];
+                // E_0 = X_0 + X_N, O_0 = X_0 - X_N
+                re[lpos(0)] = x0 + xn;
+                im[lpos(0)] = x0 - xn;
+                continue;
+            }
+            int kk = N - k;
+            cpx<T> Xk = {a[2 * k], a[2 * k + 1]};
+            cpx<T> Xc = {a[2 * kk], -a[2 * kk + 1]};  // conj X_{N-k}
+            cpx<T> E = cadd(Xk, Xc);
+            cpx<T> D = csub(Xk, Xc);
+            cpx<T> w = {rtw[k].x, -rtw[k].y};  // exp(+2 pi i k/2N)
+            cpx<T> O = cmul(D, w);
+            // Z_k = E + i O
+            re[lpos(k)] = E.x - O.y;
+            im[lpos(k)] = E.y + O.x;
+            if (kk != k) {
+                // E_{N-k} = conj E_k and O_{N-k} = conj O_k, so
+                // Z_{N-k} = conj E + i conj O
+                re[lpos(kk)] = E.x + O.y;
+                im[lpos(kk)] = O.x - E.y;
+            }
+        }
+    }
+    __device__ void post(int64_t, int64_t, const T*, const T*, int, int) const {}
+};
+
+}  // namespace jt

@@ -1,0 +1,59 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, and maps the reference's IllegalArgumentException cases
+before touching the device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import jtransforms_amd as jt
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "jtransforms_amd.h")
+
+
+def header_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(jt_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_matches_python_list():
+    assert header_symbols() == sorted(jt.EXPORTED_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(jt.LIB_PATH)
+    for name in header_symbols():
+        assert hasattr(L, name), name
+
+
+def test_version_and_status_strings():
+    L = jt.lib()
+    assert b"gfx950" in L.jt_version()
+    assert L.jt_status_string(jt.JT_NOT_POW2) == b"sizes must be power of two numbers"
+
+
+@pytest.mark.parametrize("ctor,args,msg", [
+    (jt.DoubleFFT_1D, (0,), "n must be greater than 0"),          # DoubleFFT_1D.java:119-121
+    (jt.FloatFFT_1D, (-3,), "n must be greater than 0"),
+    (jt.DoubleFFT_2D, (1, 8), "rows and columns must be greater than 1"),   # DoubleFFT_2D.java:77-79
+    (jt.DoubleFFT_3D, (4, 1, 4), "slices, rows and columns must be greater than 1"),
+    (jt.DoubleDCT_1D, (0,), "n must be greater than 0"),
+    (jt.DoubleDCT_1D, ((1 << 28) + 1,), "n must be smaller or equal to 268435456"),  # DoubleDCT_1D.java:98-100
+])
+def test_illegal_argument(ctor, args, msg):
+    with pytest.raises(jt.IllegalArgumentException, match=re.escape(msg)):
+        ctor(*args)
+
+
+def test_split_fp16_needs_f32():
+    with pytest.raises(ValueError):
+        jt.FFTBatch1D(4096, 4, precision="f64", split_fp16=True)
+
+
+def test_null_plan_rejected():
+    L = jt.lib()
+    assert L.jt_execute(None, 0, None, 0, 0, None) == jt.JT_BAD_ARG
+    assert L.jt_execute_host(None, 0, None, 0, 0) == jt.JT_BAD_ARG
+    assert L.jt_plan_elements(None, 0) == -1
