@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark of the transform hot path on one MI355X.
+
+Workload (the 2D configuration the headline metric is quoted on):
+DoubleFFT_2D(4096, 4096) -- complexForward on a 4096x4096 FP64 interleaved
+complex array plus realForward on a 4096x4096 FP64 real array, both in place,
+entries i.i.d. uniform[-1, 1).  One step = those two calls.  Every step works
+on its own freshly generated input copies (resident in HBM before the timed
+region), so no step transforms the output of an earlier one.
+
+Prints one JSON line: GFLOP/s with the 5 N log2 N (complex) / 2.5 N log2 N
+(real input) flop convention, the roofline of the dominant kernel (its
+per-launch duration measured live with HIP events on its launch stream), and
+a CPU baseline: the oracle's C port of the transforms timed on this host's
+cores over a bounded sample.
+"""
+import argparse
+import json
+import math
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = ("GFLOP/s (5Nlog2N) & %HBM peak: FP64 plain vs split-FP16 MFMA @ rel-ℓ2≤1e-6")
+HBM_PEAK_GBS = 8000.0
+ROWS = COLS = 4096
+
+
+def flops_per_step():
+    n = ROWS * COLS
+    return 5.0 * n * math.log2(n) + 2.5 * n * math.log2(n)
+
+
+def parse_kernel(name):
+    """(precision bytes, N, lines) of a stockham/generic launch label."""
+    m = re.match(r"(stockham|generic)<(f64|f32),N=(\d+),.*lines=(\d+),", name)
+    if not m:
+        return None
+    return (8 if m.group(2) == "f64" else 4), int(m.group(3)), int(m.group(4))
+
+
+def alg_bytes(name):
+    """Algorithmic HBM bytes of one launch: every point read once and written
+    once (N complex points per line, or 2N reals for the real-data tags)."""
+    p = parse_kernel(name)
+    if p is None:
+        return None
+    es, n, lines = p
+    return 2.0 * lines * n * 2 * es
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None
+    ent = data.get("kernels", {}).get(kernel)
+    if not ent:
+        return None
+    return ent.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(budget_s):
+    """The oracle's C port (OpenMP over lines) on the same workload shape."""
+    import numpy as np
+    from oracle import oracle as orc
+    threads = orc.num_threads()
+    x = orc.jrandom_doubles(20110602, 2 * ROWS * COLS)
+    y = orc.jrandom_doubles(20110603, ROWS * COLS)
+    best = float("inf")
+    runs = 0
+    t_begin = time.perf_counter()
+    while runs < 2 or (time.perf_counter() - t_begin < budget_s and runs < 50):
+        t0 = time.perf_counter()
+        orc.fftnd_complex([ROWS, COLS], x, -1)
+        orc.fft2d_real_forward(ROWS, COLS, y)
+        dt = time.perf_counter() - t0
+        best = min(best, dt)
+        runs += 1
+    del np
+    return {
+        "value": flops_per_step() / best / 1e9,
+        "unit": "GFLOP/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"full step (complexForward + realForward 4096x4096 FP64), {runs} runs, "
+                   f"minimum time {best * 1e3:.1f} ms, OpenMP threads={threads}; JTransforms "
+                   f"itself needs a JVM, absent on this host"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the transforms "
+              "here are single-device and no multi-GPU figure is produced", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import jtransforms_amd as jt
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    plan = jt.DoubleFFT_2D(ROWS, COLS)
+    g = torch.Generator(device=dev)
+    g.manual_seed(20110602)
+    base_c = torch.rand(2 * ROWS * COLS, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    base_r = torch.rand(ROWS * COLS, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    total = args.warmup + args.steps
+    bufs = [(base_c.clone(), base_r.clone()) for _ in range(total)]
+    torch.cuda.synchronize()
+
+    for i in range(args.warmup):
+        plan.complexForward(bufs[i][0])
+        plan.realForward(bufs[i][1])
+    torch.cuda.synchronize()
+
+    jt.profile_begin()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.warmup, total):
+        plan.complexForward(bufs[i][0])
+        plan.realForward(bufs[i][1])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = jt.profile_end()
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = flops_per_step() / (ms_per_step * 1e-3) / 1e9
+
+    # dominant kernel: the instantiation (launch label without its line
+    # count, i.e. what rocprofv3 groups as one kernel) with the largest total
+    # time; its algorithmic bytes and duration are averaged over its launches
+    fam = {}
+    for k, (c, ms) in prof.items():
+        key = re.sub(r",lines=\d+", "", k)
+        f = fam.setdefault(key, [0, 0.0, 0.0])
+        f[0] += c
+        f[1] += ms
+        f[2] += (alg_bytes(k) or 0.0) * c
+    dom_name, (dom_launches, dom_ms, dom_bytes) = max(fam.items(), key=lambda kv: kv[1][1])
+    dom_avg_s = dom_ms / dom_launches * 1e-3
+    bytes_launch = dom_bytes / dom_launches if dom_bytes else None
+    achieved = bytes_launch / dom_avg_s / 1e9 if bytes_launch else None
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom_name,
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+        "traffic": load_traffic(dom_name),
+        "avg_launch_us": dom_avg_s * 1e6,
+        "alg_bytes_per_launch": bytes_launch,
+    }
+    kernels = {k: {"launches": c, "avg_us": ms / c * 1e3,
+                   "GBps": (alg_bytes(k) / (ms / c * 1e-3) / 1e9) if alg_bytes(k) else None}
+               for k, (c, ms) in sorted(prof.items())}
+    # whole-step HBM rate: each step moves 2 passes x (256 MiB + 128 MiB) x read+write
+    step_bytes = 2 * 2 * (2 * ROWS * COLS * 8 + ROWS * COLS * 8)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: uniform[-1,1) FP64 from torch's generator, seed 20110602",
+        "config": {
+            "workload": "DoubleFFT_2D 4096x4096 FP64: complexForward + realForward, in place",
+            "rows": ROWS,
+            "columns": COLS,
+            "flops_per_step": flops_per_step(),
+            "hbm_bytes_per_step_alg": step_bytes,
+            "step_GBps": step_bytes / (ms_per_step * 1e-3) / 1e9,
+        },
+        "roofline": roofline,
+        "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -128,6 +128,15 @@ const char* jt_status_string(int status);
 /* Text of the last error raised on the calling host thread ("" if none). */
 const char* jt_last_error(void);
 
+/* Per-kernel timing with HIP events recorded on each launch's stream (no
+ * counterpart in the reference; used by the benchmark).  jt_profile_begin
+ * clears and enables the recorder; jt_profile_end disables it, waits for the
+ * recorded events and writes "kernel<TAB>launches<TAB>total_ms" lines into
+ * buf (NUL-terminated, truncated to len); returns the full text length.
+ * Later calls return the same text until the next jt_profile_begin. */
+int jt_profile_begin(void);
+int64_t jt_profile_end(char* buf, int64_t len);
+
 /* Library version string. */
 const char* jt_version(void);
 

@@ -39,7 +39,7 @@ EXPORTED_SYMBOLS = [
     "jt_plan_create_fft1d", "jt_plan_create_fft2d", "jt_plan_create_fft3d", "jt_plan_create_dct",
     "jt_plan_create_dst", "jt_plan_create_dht", "jt_plan_create_fft1d_batch", "jt_plan_destroy",
     "jt_execute", "jt_execute_host", "jt_plan_elements", "jt_status_string", "jt_last_error",
-    "jt_version",
+    "jt_version", "jt_profile_begin", "jt_profile_end",
 ]
 
 
@@ -91,6 +91,9 @@ def lib():
         L.jt_last_error.restype = ctypes.c_char_p
         L.jt_version.argtypes = []
         L.jt_version.restype = ctypes.c_char_p
+        L.jt_profile_begin.argtypes = []
+        L.jt_profile_end.argtypes = [ctypes.c_char_p, i64]
+        L.jt_profile_end.restype = i64
         _lib = L
     return _lib
 
@@ -300,6 +303,26 @@ DoubleDHT_2D = _r2r_class("DoubleDHT_2D", "jt_plan_create_dht", JT_F64, has_scal
 FloatDHT_2D = _r2r_class("FloatDHT_2D", "jt_plan_create_dht", JT_F32, has_scale_fwd=False)
 DoubleDHT_3D = _r2r_class("DoubleDHT_3D", "jt_plan_create_dht", JT_F64, has_scale_fwd=False)
 FloatDHT_3D = _r2r_class("FloatDHT_3D", "jt_plan_create_dht", JT_F32, has_scale_fwd=False)
+
+
+def profile_begin():
+    """Start recording per-kernel HIP-event timings (jt_profile_begin)."""
+    rc = lib().jt_profile_begin()
+    if rc:
+        _raise(rc)
+
+
+def profile_end():
+    """Stop recording; returns {kernel: (launches, total_ms)}."""
+    L = lib()
+    n = L.jt_profile_end(None, 0)
+    buf = ctypes.create_string_buffer(int(n) + 1)
+    L.jt_profile_end(buf, n + 1)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms = line.split("\t")
+        out[name] = (int(cnt), float(ms))
+    return out
 
 
 def build():

@@ -198,7 +198,9 @@ void run_real_forward_nd(jt_plan* p, T* a, hipStream_t st) {
         const int64_t r = p->dims[0], c = p->dims[1], ch = c / 2;
         real_forward<T>(*I.real, a, r, ws, st);
         axis_c2c<T, -1>(*I.ax[0], z, LineMap{ch, ch, 1, 0, ch}, T(1), ws, st);
+        { JT_PROF(st, "rdft2d_fix");
         hipLaunchKernelGGL((rdft2d_fix<T>), ew_grid(r / 2), dim3(256), 0, st, a, r, c, 1);
+        }
         JT_HIP(hipGetLastError());
         return;
     }
@@ -206,7 +208,9 @@ void run_real_forward_nd(jt_plan* p, T* a, hipStream_t st) {
     real_forward<T>(*I.real, a, S * r, ws, st);
     axis_c2c<T, -1>(*I.ax[1], z, LineMap{S * ch, ch, 1, r * ch, ch}, T(1), ws, st);
     axis_c2c<T, -1>(*I.ax[0], z, LineMap{r * ch, r * ch, 1, 0, r * ch}, T(1), ws, st);
+    { JT_PROF(st, "rdft3d_fix");
     hipLaunchKernelGGL((rdft3d_fix<T>), ew_grid(S * (r / 2 + 1)), dim3(256), 0, st, a, S, r, c, 1);
+    }
     JT_HIP(hipGetLastError());
 }
 
@@ -217,14 +221,18 @@ void run_real_inverse_nd(jt_plan* p, T* a, bool scale, hipStream_t st) {
     cpx<T>* z = reinterpret_cast<cpx<T>*>(a);
     if (p->ndim == 2) {
         const int64_t r = p->dims[0], c = p->dims[1], ch = c / 2;
+        { JT_PROF(st, "rdft2d_fix");
         hipLaunchKernelGGL((rdft2d_fix<T>), ew_grid(r / 2), dim3(256), 0, st, a, r, c, -1);
+        }
         JT_HIP(hipGetLastError());
         axis_c2c<T, 1>(*I.ax[0], z, LineMap{ch, ch, 1, 0, ch}, scale ? T(1.0 / (double)r) : T(1), ws, st);
         real_inverse<T>(*I.real, a, r, scale, ws, st);
         return;
     }
     const int64_t S = p->dims[0], r = p->dims[1], c = p->dims[2], ch = c / 2;
+    { JT_PROF(st, "rdft3d_fix");
     hipLaunchKernelGGL((rdft3d_fix<T>), ew_grid(S * (r / 2 + 1)), dim3(256), 0, st, a, S, r, c, -1);
+    }
     JT_HIP(hipGetLastError());
     axis_c2c<T, 1>(*I.ax[0], z, LineMap{r * ch, r * ch, 1, 0, r * ch}, scale ? T(1.0 / (double)S) : T(1),
                    ws, st);
@@ -241,10 +249,14 @@ void run_real_full_nd(jt_plan* p, T* a, bool scale, hipStream_t st) {
     const int64_t tot = total(p);
     // expand in place from the back: reals -> complex (use workspace copy)
     cpx<T>* ws = I.ws.template as<cpx<T>>();
+    { JT_PROF(st, "real_to_complex");
     hipLaunchKernelGGL((real_to_complex<T>), ew_grid(tot), dim3(256), 0, st, a, ws, tot, (int64_t)1);
+    }
     JT_HIP(hipGetLastError());
+    { JT_PROF(st, "copy_complex");
     hipLaunchKernelGGL((copy_complex<T>), ew_grid(tot), dim3(256), 0, st, ws, reinterpret_cast<cpx<T>*>(a),
                        tot);
+    }
     JT_HIP(hipGetLastError());
     run_complex_nd<T, DIR>(p, reinterpret_cast<cpx<T>*>(a), scale, st);
 }
@@ -421,6 +433,33 @@ const char* jt_status_string(int s) {
 }
 
 const char* jt_last_error(void) { return g_err.c_str(); }
+
+int jt_profile_begin(void) {
+    return guarded([&]() -> int {
+        prof_clear();
+        prof_on().store(true);
+        return JT_OK;
+    });
+}
+
+int64_t jt_profile_end(char* buf, int64_t len) {
+    // the summary of the last recording stays readable until the next begin
+    static std::string last;
+    if (prof_on().exchange(false)) {
+        try {
+            last = prof_summary();
+            prof_clear();
+        } catch (...) {
+            return -1;
+        }
+    }
+    if (buf && len > 0) {
+        int64_t k = std::min<int64_t>(len - 1, (int64_t)last.size());
+        std::memcpy(buf, last.data(), (size_t)k);
+        buf[k] = 0;
+    }
+    return (int64_t)last.size();
+}
 
 const char* jt_version(void) { return "jtransforms-amd 0.1 (gfx950)"; }
 

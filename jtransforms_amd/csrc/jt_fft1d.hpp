@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "jt_generic.hpp"
+#include "jt_prof.hpp"
 #include "jt_io.hpp"
 #include "jt_stockham.hpp"
 
@@ -331,6 +332,9 @@ void go_tpl(IO io, const cpx<T>* tw, hipStream_t st) {
     constexpr bool SPLIT = S::split && !IO::pre_lds && !IO::post_lds;
     int64_t groups = (io.nlines + W - 1) / W;
     if (groups <= 0) return;
+    JT_PROF(st, std::string("stockham<") + (sizeof(T) == 8 ? "f64" : "f32") + ",N=" + std::to_string(N) +
+                    (COL ? ",col" : ",row") + ",W=" + std::to_string(W) + ",lines=" + std::to_string(io.nlines) +
+                    "," + IO::tag() + ">");
     hipLaunchKernelGGL((stockham_kernel<T, DIR, N, TL, W, COL, SPLIT, IO, Rs...>), dim3((unsigned)groups),
                        dim3(TL * W), 0, st, io, tw);
     JT_HIP(hipGetLastError());
@@ -367,6 +371,8 @@ void go_gen(const Fft1D<T>& p, IO io, hipStream_t st) {
         attr_set = true;
     }
     int64_t lines = io.nlines;
+    JT_PROF(st, std::string("generic<") + (sizeof(T) == 8 ? "f64" : "f32") + ",N=" + std::to_string(n) +
+                    ",lines=" + std::to_string(lines) + "," + IO::tag() + ">");
     unsigned gx = (unsigned)(lines < 65535 ? lines : 65535);
     unsigned gy = (unsigned)((lines + gx - 1) / gx);
     hipLaunchKernelGGL((generic_kernel<T, DIR, IO>), dim3(gx, gy), dim3(nt), shm, st, io,
@@ -484,16 +490,22 @@ void c2c_contig(const Fft1D<T>& p, const cpx<T>* in, cpx<T>* out, int64_t lines,
     cpx<T>* y = ws;
     cpx<T>* ws2 = ws + lines * p.m;
     const cpx<T>* ch = p.chirp.template as<cpx<T>>();
+    { JT_PROF(st, "blue_premul");
     hipLaunchKernelGGL((blue_premul<T, DIR>), ew_grid(lines * p.m), dim3(256), 0, st, in, y, ch, p.n, p.m,
                        lines);
+    }
     JT_HIP(hipGetLastError());
     c2c_contig<T, -1>(*p.pm, y, y, lines, T(1), ws2, st);
+    { JT_PROF(st, "blue_pointwise");
     hipLaunchKernelGGL((blue_pointwise<T, DIR>), ew_grid(lines * p.m), dim3(256), 0, st, y,
                        p.bspec.template as<cpx<T>>(), p.m, lines);
+    }
     JT_HIP(hipGetLastError());
     c2c_contig<T, 1>(*p.pm, y, y, lines, T(1), ws2, st);
+    { JT_PROF(st, "blue_postmul");
     hipLaunchKernelGGL((blue_postmul<T, DIR>), ew_grid(lines * p.n), dim3(256), 0, st, y, out, ch, p.n,
                        p.m, lines, scale);
+    }
     JT_HIP(hipGetLastError());
 }
 

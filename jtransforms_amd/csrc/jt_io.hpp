@@ -18,6 +18,7 @@ struct LineMap {
 template <typename T>
 struct IOStrided {
     static constexpr bool pre_lds = false, post_lds = false;
+    static const char* tag() { return "c2c"; }
     const cpx<T>* in;
     cpx<T>* out;
     LineMap m;
@@ -49,6 +50,7 @@ __device__ __forceinline__ cpx<T> tw2(const cpx<T>* __restrict__ hi, const cpx<T
 template <typename T, int DIR>
 struct IOFourA {
     static constexpr bool pre_lds = false, post_lds = false;
+    static const char* tag() { return "fourA"; }
     const cpx<T>* in;
     cpx<T>* out;
     const cpx<T>* hi;
@@ -71,6 +73,7 @@ struct IOFourA {
 template <typename T>
 struct IOFourB {
     static constexpr bool pre_lds = false, post_lds = false;
+    static const char* tag() { return "fourB"; }
     const cpx<T>* in;
     cpx<T>* out;
     int64_t nlines, n, n1, n2;
@@ -96,6 +99,7 @@ struct IOFourB {
 template <typename T>
 struct IOR2C {
     static constexpr bool pre_lds = false, post_lds = true;
+    static const char* tag() { return "r2c"; }
     const cpx<T>* in;  // reals viewed as complex pairs
     T* out;            // packed reals
     const cpx<T>* rtw;
@@ -144,6 +148,7 @@ struct IOR2C {
 template <typename T>
 struct IOC2R {
     static constexpr bool pre_lds = true, post_lds = false;
+    static const char* tag() { return "c2r"; }
     const T* in;     // packed reals
     cpx<T>* out;     // reals viewed as complex pairs
     const cpx<T>* rtw;

@@ -45,10 +45,14 @@ void axis_c2c(const Fft1D<T>& p, cpx<T>* data, LineMap lm, T scale, cpx<T>* ws, 
     }
     cpx<T>* g = ws;
     cpx<T>* ws2 = ws + lm.nlines * p.n;
+    { JT_PROF(st, "gather_lines");
     hipLaunchKernelGGL((gather_lines<T>), ew_grid(lm.nlines * p.n), dim3(256), 0, st, data, g, lm, p.n);
+    }
     JT_HIP(hipGetLastError());
     c2c_contig<T, DIR>(p, g, g, lm.nlines, scale, ws2, st);
+    { JT_PROF(st, "scatter_lines");
     hipLaunchKernelGGL((scatter_lines<T>), ew_grid(lm.nlines * p.n), dim3(256), 0, st, g, data, lm, p.n);
+    }
     JT_HIP(hipGetLastError());
 }
 

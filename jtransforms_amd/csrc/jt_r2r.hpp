@@ -53,6 +53,9 @@ inline R2RScale dct3_scale(int64_t n, bool scale) {
 template <typename T, int KIND, bool FWD>
 struct IOR2R {
     static constexpr bool pre_lds = true, post_lds = true;
+    static const char* tag() {
+        return KIND == R2R_DHT ? "dht" : (KIND == R2R_DCT ? (FWD ? "dct2" : "dct3") : (FWD ? "dst2" : "dst3"));
+    }
     T* data;
     const cpx<T>* q;
     const cpx<T>* rtw;
@@ -363,16 +366,20 @@ void r2r_axis_k(const R2R1D<T>& r, T* data, LineMap lm, bool scale, cpx<T>* ws, 
     const int64_t L = r.L();
     cpx<T>* y = ws;
     cpx<T>* ws2 = ws + lm.nlines * L;
+    { JT_PROF(st, "r2r_fb_pre");
     hipLaunchKernelGGL((r2r_fb_pre<T, KIND, FWD>), ew_grid(lm.nlines * L), dim3(256), 0, st, data, y, lm, n, L,
                        r.q.template as<cpx<T>>(), T(sc.f0));
+    }
     JT_HIP(hipGetLastError());
     if (KIND == R2R_DHT || FWD) c2c_contig<T, -1>(*r.big, y, y, lm.nlines, T(1), ws2, st);
     else c2c_contig<T, 1>(*r.big, y, y, lm.nlines, T(1), ws2, st);
     // inverse: backward 2n sum of u gives y(a') directly (the 1/2 on a_0 is
     // in u_0): fb_post multiplies by 2*g*0.5 -> use g/2 here
     T gpost = (KIND == R2R_DHT || FWD) ? T(sc.g) : T(sc.g * 0.5);
+    { JT_PROF(st, "r2r_fb_post");
     hipLaunchKernelGGL((r2r_fb_post<T, KIND, FWD>), ew_grid(lm.nlines * n), dim3(256), 0, st, y, data, lm, n, L,
                        r.q.template as<cpx<T>>(), T(sc.f0), gpost);
+    }
     JT_HIP(hipGetLastError());
 }
 
@@ -403,7 +410,9 @@ void run_r2r_nd(int family, int nd, const int64_t* d, const std::unique_ptr<R2R1
         r2r_axis<T>(family, *r[1], a, LineMap{R, R, C, 0, 1}, fwd, scale, ws, st);
         r2r_axis<T>(family, *r[0], a, LineMap{C, C, 1, 0, C}, fwd, scale, ws, st);
         if (family == 3) {
+            { JT_PROF(st, "ytransform2d");
             hipLaunchKernelGGL((ytransform2d<T>), ew_grid((R / 2 + 1) * (C / 2 + 1)), dim3(256), 0, st, a, R, C);
+            }
             JT_HIP(hipGetLastError());
         }
         return;
@@ -413,8 +422,10 @@ void run_r2r_nd(int family, int nd, const int64_t* d, const std::unique_ptr<R2R1
     r2r_axis<T>(family, *r[1], a, LineMap{S * C, C, 1, R * C, C}, fwd, scale, ws, st);
     r2r_axis<T>(family, *r[0], a, LineMap{R * C, R * C, 1, 0, R * C}, fwd, scale, ws, st);
     if (family == 3) {
+        { JT_PROF(st, "ytransform3d");
         hipLaunchKernelGGL((ytransform3d<T>), ew_grid((S / 2 + 1) * (R / 2 + 1) * (C / 2 + 1)), dim3(256), 0, st,
                            a, S, R, C);
+        }
         JT_HIP(hipGetLastError());
     }
 }

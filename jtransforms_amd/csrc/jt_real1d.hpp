@@ -169,8 +169,10 @@ void real_forward(const Real1D<T>& r, T* a, int64_t lines, cpx<T>* ws, hipStream
             run_single<T, -1, false>(*r.half, io, st);
         } else {
             c2c_contig<T, -1>(*r.half, z, z, lines, T(1), ws, st);
+            { JT_PROF(st, "r2c_post_inplace");
             hipLaunchKernelGGL((r2c_post_inplace<T>), ew_grid(lines * (N / 2 + 1)), dim3(256), 0, st, z,
                                r.rtw.template as<cpx<T>>(), N, lines, T(1));
+            }
             JT_HIP(hipGetLastError());
         }
         return;
@@ -178,10 +180,14 @@ void real_forward(const Real1D<T>& r, T* a, int64_t lines, cpx<T>* ws, hipStream
     // odd n: full complex transform of the real input, then pack
     cpx<T>* X = ws;
     cpx<T>* ws2 = ws + lines * n;
+    { JT_PROF(st, "real_to_complex");
     hipLaunchKernelGGL((real_to_complex<T>), ew_grid(lines * n), dim3(256), 0, st, a, X, n, lines);
+    }
     JT_HIP(hipGetLastError());
     c2c_contig<T, -1>(*r.full, X, X, lines, T(1), ws2, st);
+    { JT_PROF(st, "pack_real");
     hipLaunchKernelGGL((pack_real<T>), ew_grid(lines * n), dim3(256), 0, st, X, a, n, lines);
+    }
     JT_HIP(hipGetLastError());
 }
 
@@ -206,8 +212,10 @@ void real_inverse(const Real1D<T>& r, T* a, int64_t lines, bool scale, cpx<T>* w
                         false};
             run_single<T, 1, false>(*r.half, io, st);
         } else {
+            { JT_PROF(st, "c2r_pre_inplace");
             hipLaunchKernelGGL((c2r_pre_inplace<T>), ew_grid(lines * (N / 2 + 1)), dim3(256), 0, st, z,
                                r.rtw.template as<cpx<T>>(), N, lines);
+            }
             JT_HIP(hipGetLastError());
             c2c_contig<T, 1>(*r.half, z, z, lines, f, ws, st);
         }
@@ -215,12 +223,16 @@ void real_inverse(const Real1D<T>& r, T* a, int64_t lines, bool scale, cpx<T>* w
     }
     cpx<T>* X = ws;
     cpx<T>* ws2 = ws + lines * n;
+    { JT_PROF(st, "unpack_real");
     hipLaunchKernelGGL((unpack_real<T>), ew_grid(lines * n), dim3(256), 0, st, a, X, n, lines);
+    }
     JT_HIP(hipGetLastError());
     c2c_contig<T, 1>(*r.full, X, X, lines, T(1), ws2, st);
     // the backward sum is n x; unscaled non-power-of-two result is n x
+    { JT_PROF(st, "complex_real_part");
     hipLaunchKernelGGL((complex_real_part<T>), ew_grid(lines * n), dim3(256), 0, st, X, a, lines * n,
                        scale ? T(1.0 / (double)n) : T(1));
+    }
     JT_HIP(hipGetLastError());
 }
 
@@ -234,12 +246,16 @@ void real_full(const Real1D<T>& r, T* a, int64_t lines, bool scale, cpx<T>* ws, 
     // (one line per call there), outputs 2n reals apart
     cpx<T>* X = ws;
     cpx<T>* ws2 = ws + lines * n;
+    { JT_PROF(st, "real_to_complex");
     hipLaunchKernelGGL((real_to_complex<T>), ew_grid(lines * n), dim3(256), 0, st, a, X, n, lines);
+    }
     JT_HIP(hipGetLastError());
     T s = (DIR > 0 && scale) ? T(1.0 / (double)n) : T(1);
     c2c_contig<T, DIR>(*r.full, X, X, lines, s, ws2, st);
+    { JT_PROF(st, "copy_complex");
     hipLaunchKernelGGL((copy_complex<T>), ew_grid(lines * n), dim3(256), 0, st, X,
                        reinterpret_cast<cpx<T>*>(a), lines * n);
+    }
     JT_HIP(hipGetLastError());
 }
 
