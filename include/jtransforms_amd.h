@@ -102,6 +102,11 @@ int jt_plan_create_dht(jt_plan** out, int precision, int ndim, const int64_t* di
 int jt_plan_create_fft1d_batch(jt_plan** out, int precision, int64_t n, int64_t batch,
                                uint32_t flags);
 
+/* Sets plan flags after creation (JT_FLAG_SPLIT_FP16 on FP32 FFT plans of
+ * any rank, e.g. a FloatFFT_3D).  No counterpart in the reference: selects
+ * the split-FP16 MFMA form of the radix-16 stages. */
+int jt_plan_set_flags(jt_plan* plan, uint32_t flags);
+
 /* Frees the plan's tables and workspace (the reference's plans are garbage
  * collected). */
 int jt_plan_destroy(jt_plan* plan);

@@ -39,7 +39,7 @@ EXPORTED_SYMBOLS = [
     "jt_plan_create_fft1d", "jt_plan_create_fft2d", "jt_plan_create_fft3d", "jt_plan_create_dct",
     "jt_plan_create_dst", "jt_plan_create_dht", "jt_plan_create_fft1d_batch", "jt_plan_destroy",
     "jt_execute", "jt_execute_host", "jt_plan_elements", "jt_status_string", "jt_last_error",
-    "jt_version", "jt_profile_begin", "jt_profile_end",
+    "jt_version", "jt_profile_begin", "jt_profile_end", "jt_plan_set_flags",
 ]
 
 
@@ -79,6 +79,7 @@ def lib():
             getattr(L, f).argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(i64)]
         L.jt_plan_create_fft1d_batch.argtypes = [P, ctypes.c_int, i64, i64, ctypes.c_uint32]
         L.jt_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.jt_plan_set_flags.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
         L.jt_execute.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, i64, ctypes.c_int,
                                  ctypes.c_void_p]
         L.jt_execute_host.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, i64,
@@ -179,6 +180,12 @@ def _offa_scale(args, name):
 
 
 class _FFTMethods(_Plan):
+    def set_split_fp16(self, on: bool = True):
+        """Select the split-FP16 MFMA form of the radix-16 stages (FP32 plans)."""
+        rc = lib().jt_plan_set_flags(self._h, JT_FLAG_SPLIT_FP16 if on else 0)
+        if rc:
+            _raise(rc)
+
     def complexForward(self, a, offa: int = 0):
         self._run(COMPLEX_FORWARD, a, offa)
 

@@ -97,6 +97,14 @@ int64_t ws_need(jt_plan* p) {
     return w;
 }
 
+// split-FP16 MFMA radix-16 stages on every compile-time complex axis plan
+template <typename T>
+void apply_flags(Impl<T>& I, uint32_t flags) {
+    const bool mf = sizeof(T) == 4 && (flags & JT_FLAG_SPLIT_FP16);
+    for (int a = 0; a < 3; a++)
+        if (I.ax[a]) I.ax[a]->mf = mf && I.ax[a]->kind == K_TPL;
+}
+
 template <typename T>
 void build(jt_plan* p) {
     auto I = std::make_unique<Impl<T>>();
@@ -114,6 +122,7 @@ void build(jt_plan* p) {
     } else {
         for (int a = 0; a < p->ndim; a++) I->r2r[a] = make_r2r1d<T>(p->family, p->dims[a]);
     }
+    apply_flags(*I, p->flags);
     if constexpr (sizeof(T) == 8) p->d = std::move(I);
     else p->f = std::move(I);
     int64_t w = ws_need<T>(p);
@@ -360,6 +369,19 @@ int jt_plan_create_fft1d_batch(jt_plan** out, int precision, int64_t n, int64_t 
     if ((flags & JT_FLAG_SPLIT_FP16) && precision != JT_F32)
         return set_err(JT_BAD_ARG, "split-FP16 form is defined for FP32 plans only");
     return create(out, FAM_FFT, precision, 1, &n, batch, flags);
+}
+
+int jt_plan_set_flags(jt_plan* p, uint32_t flags) {
+    return guarded([&]() -> int {
+        if (!p) return set_err(JT_BAD_ARG, "null plan");
+        if (flags & ~(uint32_t)JT_FLAG_SPLIT_FP16) return set_err(JT_BAD_ARG, "unknown flags");
+        if ((flags & JT_FLAG_SPLIT_FP16) && p->precision != JT_F32)
+            return set_err(JT_BAD_ARG, "split-FP16 form is defined for FP32 plans only");
+        if (p->family != FAM_FFT) return set_err(JT_BAD_ARG, "flags apply to FFT plans");
+        p->flags = flags;
+        apply_flags(*p->f, flags);
+        return JT_OK;
+    });
 }
 
 int jt_plan_destroy(jt_plan* plan) {

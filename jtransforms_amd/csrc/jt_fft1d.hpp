@@ -15,6 +15,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "jt_generic.hpp"
@@ -212,6 +213,7 @@ template <typename T>
 struct Fft1D {
     int64_t n = 0;
     int kind = K_GEN;
+    bool mf = false;  // split-FP16 MFMA radix-16 stages (FP32 compile-time plans)
     std::vector<int> radices;
     GenPlan gp{};
     DevBuf tw;
@@ -325,35 +327,44 @@ struct TplShape {
     static constexpr bool split = bytes_full > 65536;
 };
 
-template <typename T, int DIR, int N, int TL, bool COL, class IO, int... Rs>
+template <typename T, int DIR, int N, int TL, bool COL, bool MF, class IO, int... Rs>
 void go_tpl(IO io, const cpx<T>* tw, hipStream_t st) {
     using S = TplShape<T, N, TL, COL>;
     constexpr int W = S::W;
     constexpr bool SPLIT = S::split && !IO::pre_lds && !IO::post_lds;
+    static_assert(!MF || (W * TL) % 64 == 0, "MFMA stages need whole waves");
     int64_t groups = (io.nlines + W - 1) / W;
     if (groups <= 0) return;
     JT_PROF(st, std::string("stockham<") + (sizeof(T) == 8 ? "f64" : "f32") + ",N=" + std::to_string(N) +
-                    (COL ? ",col" : ",row") + ",W=" + std::to_string(W) + ",lines=" + std::to_string(io.nlines) +
+ (COL ? ",col" : ",row") + (MF ? ",mfma" : "") + ",W=" + std::to_string(W) + ",lines=" + std::to_string(io.nlines) +
                     "," + IO::tag() + ">");
-    hipLaunchKernelGGL((stockham_kernel<T, DIR, N, TL, W, COL, SPLIT, IO, Rs...>), dim3((unsigned)groups),
+    hipLaunchKernelGGL((stockham_kernel<T, DIR, N, TL, W, COL, SPLIT, MF, IO, Rs...>), dim3((unsigned)groups),
                        dim3(TL * W), 0, st, io, tw);
     JT_HIP(hipGetLastError());
 }
 
-template <typename T, int DIR, bool COL, class IO>
-bool dispatch_tpl(int64_t n, IO io, const cpx<T>* tw, hipStream_t st) {
+template <typename T, int DIR, bool COL, bool MF, class IO>
+bool dispatch_tpl_mf(int64_t n, IO io, const cpx<T>* tw, hipStream_t st) {
     switch (n) {
-        case 4096: go_tpl<T, DIR, 4096, 256, COL, IO, 16, 16, 16>(io, tw, st); return true;
-        case 2048: go_tpl<T, DIR, 2048, 128, COL, IO, 16, 16, 8>(io, tw, st); return true;
-        case 1024: go_tpl<T, DIR, 1024, 64, COL, IO, 16, 16, 4>(io, tw, st); return true;
-        case 512: go_tpl<T, DIR, 512, 64, COL, IO, 8, 8, 8>(io, tw, st); return true;
-        case 256: go_tpl<T, DIR, 256, 16, COL, IO, 16, 16>(io, tw, st); return true;
-        case 128: go_tpl<T, DIR, 128, 8, COL, IO, 16, 8>(io, tw, st); return true;
-        case 64: go_tpl<T, DIR, 64, 8, COL, IO, 8, 8>(io, tw, st); return true;
-        case 32: go_tpl<T, DIR, 32, 4, COL, IO, 8, 4>(io, tw, st); return true;
-        case 16: go_tpl<T, DIR, 16, 1, COL, IO, 16>(io, tw, st); return true;
+        case 4096: go_tpl<T, DIR, 4096, 256, COL, MF, IO, 16, 16, 16>(io, tw, st); return true;
+        case 2048: go_tpl<T, DIR, 2048, 128, COL, MF, IO, 16, 16, 8>(io, tw, st); return true;
+        case 1024: go_tpl<T, DIR, 1024, 64, COL, MF, IO, 16, 16, 4>(io, tw, st); return true;
+        case 512: go_tpl<T, DIR, 512, 64, COL, false, IO, 8, 8, 8>(io, tw, st); return true;
+        case 256: go_tpl<T, DIR, 256, 16, COL, MF, IO, 16, 16>(io, tw, st); return true;
+        case 128: go_tpl<T, DIR, 128, 8, COL, false, IO, 16, 8>(io, tw, st); return true;
+        case 64: go_tpl<T, DIR, 64, 8, COL, false, IO, 8, 8>(io, tw, st); return true;
+        case 32: go_tpl<T, DIR, 32, 4, COL, false, IO, 8, 4>(io, tw, st); return true;
+        case 16: go_tpl<T, DIR, 16, 1, COL, MF, IO, 16>(io, tw, st); return true;
         default: return false;
     }
+}
+
+template <typename T, int DIR, bool COL, class IO>
+bool dispatch_tpl(int64_t n, IO io, const cpx<T>* tw, hipStream_t st, bool mf = false) {
+    if constexpr (sizeof(T) == 4 && std::is_same<IO, IOStrided<T>>::value) {
+        if (mf) return dispatch_tpl_mf<T, DIR, COL, true>(n, io, tw, st);
+    }
+    return dispatch_tpl_mf<T, DIR, COL, false>(n, io, tw, st);
 }
 
 template <typename T, int DIR, class IO>
@@ -384,7 +395,7 @@ void go_gen(const Fft1D<T>& p, IO io, hipStream_t st) {
 template <typename T, int DIR, bool COL, class IO>
 void run_single(const Fft1D<T>& p, IO io, hipStream_t st) {
     if (p.kind == K_TPL) {
-        if (dispatch_tpl<T, DIR, COL>(p.n, io, p.tw.template as<cpx<T>>(), st)) return;
+        if (dispatch_tpl<T, DIR, COL>(p.n, io, p.tw.template as<cpx<T>>(), st, p.mf)) return;
     }
     if (p.kind == K_GEN || p.kind == K_TPL) {
         // generic kernel needs a radix plan

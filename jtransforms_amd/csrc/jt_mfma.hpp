@@ -1,0 +1,141 @@
+// Radix-16 butterflies as a dense DFT-matrix product on MFMA (gfx950), in
+// split FP16 with FP32 accumulation.
+//
+// A wave holds 64 butterflies (one per lane, 16 complex inputs each, twiddles
+// already applied).  They are processed as 4 tiles of 16 butterflies:
+//     X = F16 * V,  F16[i][k] = exp(DIR 2 pi i ik/16)
+// written over the reals as two 16x32 by 32x16 products:
+//     Xr = [Fr | -Fi] [Vr; Vi],   Xi = [Fi | Fr] [Vr; Vi]
+// with v_mfma_f32_16x16x32_f16.  Each operand is split as x ~ hi + lo
+// (hi = fp16(x), lo = fp16(x - hi)) and the product is the sum of the three
+// terms hi*hi + hi*lo + lo*hi accumulated in FP32 (the lo*lo term, ~2^-22
+// relative, is dropped).  The data operand is scaled per butterfly (per MFMA
+// column) by a power of two that brings its largest entry to [2^13, 2^14),
+// so fp16's range holds it and the scaling is exact; results are unscaled
+// in FP32.
+//
+// Operand layouts of the 16x16x32 f16 MFMA (checked on the GPU): lane l
+// holds A[i][k] with i = l%16, k = 8*(l/16)+v; B[k][j] with j = l%16,
+// k = 8*(l/16)+v; D[i][j] with j = l%16, i = 4*(l/16)+v.
+#pragma once
+#include "jt_dft.hpp"
+
+namespace jt {
+
+typedef _Float16 jt_half8 __attribute__((ext_vector_type(8)));
+typedef float jt_float4 __attribute__((ext_vector_type(4)));
+
+// LDS scratch per wave: 64 butterflies x 16 points, re and im, row pitch 17
+constexpr int MF_PITCH = 17;
+constexpr int MF_WAVE_FLOATS = 2 * 64 * MF_PITCH;
+
+struct MfmaDft16 {
+    jt_half8 arh, arl, aih, ail;  // A operands of Xr and Xi, hi/lo
+
+    template <int DIR>
+    __device__ __forceinline__ void init(int lane) {
+        // exact cos/sin(2 pi m/16), m = 0..15
+        const float c16[16] = {1.0f, 0.92387953251128675613f, 0.70710678118654752440f,
+                               0.38268343236508977173f, 0.0f, -0.38268343236508977173f,
+                               -0.70710678118654752440f, -0.92387953251128675613f, -1.0f,
+                               -0.92387953251128675613f, -0.70710678118654752440f,
+                               -0.38268343236508977173f, 0.0f, 0.38268343236508977173f,
+                               0.70710678118654752440f, 0.92387953251128675613f};
+        const int i = lane % 16;
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            int k = 8 * (lane / 16) + v;
+            int kk = k & 15;
+            int m = (i * kk) & 15;
+            float fr = c16[m];
+            float fi = float(DIR) * c16[(m + 12) & 15];  // sin(x) = cos(x - pi/2)
+            float ar = k < 16 ? fr : -fi;
+            float ai = k < 16 ? fi : fr;
+            _Float16 h;
+            h = (_Float16)ar;
+            arh[v] = h;
+            arl[v] = (_Float16)(ar - (float)h);
+            h = (_Float16)ai;
+            aih[v] = h;
+            ail[v] = (_Float16)(ai - (float)h);
+        }
+    }
+
+    // v: this lane's butterfly (16 complex, twiddled).  scr: the wave's LDS
+    // scratch (MF_WAVE_FLOATS floats).  On return v holds the 16 outputs.
+    __device__ __forceinline__ void run(cpx<float>* v, float* scr, int lane) const {
+        float* sre = scr;
+        float* sim = scr + 64 * MF_PITCH;
+        // 1. park the wave's butterflies in LDS, [butterfly][point]
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            sre[lane * MF_PITCH + r] = v[r].x;
+            sim[lane * MF_PITCH + r] = v[r].y;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // 2. gather the B operands of the 4 tiles: lane l, tile q holds
+        //    column j = l%16 (butterfly 16q+j), rows k = 8(l/16)+v:
+        //    l/16 in {0,1}: Re of points 8(l/16 & 1)+v; {2,3}: Im of them
+        const int j = lane % 16;
+        const int g = lane / 16;
+        const float* src = (g >= 2) ? sim : sre;
+        const int r0 = 8 * (g & 1);
+        float b[4][8];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int u = 0; u < 8; u++) b[q][u] = src[(16 * q + j) * MF_PITCH + r0 + u];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // 3. per tile: scale, split, 6 MFMAs, unscale, park outputs
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            float m = 0.0f;
+#pragma unroll
+            for (int u = 0; u < 8; u++) m = fmaxf(m, fabsf(b[q][u]));
+            m = fmaxf(m, __shfl_xor(m, 16));
+            m = fmaxf(m, __shfl_xor(m, 32));
+            int ex = 0;
+            (void)frexpf(m, &ex);  // m = f 2^ex, f in [0.5, 1)
+            const int sh = (m > 0.0f) ? (14 - ex) : 0;
+            const float s = ldexpf(1.0f, sh);
+            const float inv = ldexpf(1.0f, -sh);
+            jt_half8 bh, bl;
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                float x = b[q][u] * s;
+                _Float16 h = (_Float16)x;
+                bh[u] = h;
+                bl[u] = (_Float16)(x - (float)h);
+            }
+            jt_float4 dr = {0.f, 0.f, 0.f, 0.f}, di = {0.f, 0.f, 0.f, 0.f};
+            dr = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bh, dr, 0, 0, 0);
+            dr = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bl, dr, 0, 0, 0);
+            dr = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bh, dr, 0, 0, 0);
+            di = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bh, di, 0, 0, 0);
+            di = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bl, di, 0, 0, 0);
+            di = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bh, di, 0, 0, 0);
+            // D[i][j], i = 4*(l/16)+v: output i of butterfly 16q+j
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                int i = 4 * g + u;
+                sre[(16 * q + j) * MF_PITCH + i] = dr[u] * inv;
+                sim[(16 * q + j) * MF_PITCH + i] = di[u] * inv;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // 4. each lane takes back its butterfly's outputs
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = cpx<float>{sre[lane * MF_PITCH + r], sim[lane * MF_PITCH + r]};
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+
+}  // namespace jt
