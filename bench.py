@@ -97,6 +97,126 @@ def cpu_baseline(budget_s):
     }
 
 
+def roofline_from_profile(prof):
+    """Dominant kernel instantiation and its achieved HBM rate."""
+    fam = {}
+    for k, (c, ms) in prof.items():
+        key = re.sub(r",(lines|es)=\d+", "", k)
+        f = fam.setdefault(key, [0, 0.0, 0.0])
+        f[0] += c
+        f[1] += ms
+        f[2] += (alg_bytes(k) or 0.0) * c
+    name, (launches, ms, byt) = max(fam.items(), key=lambda kv: kv[1][1])
+    avg_s = ms / launches * 1e-3
+    per = byt / launches if byt else None
+    ach = per / avg_s / 1e9 if per else None
+    kernels = {k: {"launches": c, "avg_us": m / c * 1e3,
+                   "GBps": (alg_bytes(k) / (m / c * 1e-3) / 1e9) if alg_bytes(k) else None}
+               for k, (c, m) in sorted(prof.items())}
+    return {"bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS if ach else None, "traffic": load_traffic(name),
+            "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": per, "kernels": kernels}
+
+
+def timed(torch, jt, calls, steps, warmup):
+    """Run calls[i]() for warm-up then timed steps; returns (ms/step, profile)."""
+    for i in range(warmup):
+        calls[i]()
+    torch.cuda.synchronize()
+    jt.profile_begin()
+    t0 = time.perf_counter()
+    for i in range(warmup, warmup + steps):
+        calls[i]()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return el * 1e3 / steps, jt.profile_end()
+
+
+def other_workload(args, torch, jt):
+    """The other configurations: one JSON object each (not the headline line)."""
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(20110602)
+    K, W = args.steps, args.warmup
+    out = {"workload": args.workload, "steps": K, "warmup": W}
+
+    def rel(a, b):
+        return float(torch.linalg.vector_norm((a.double() - b.double())) / torch.linalg.vector_norm(b.double()))
+
+    if args.workload in ("batch4096", "fft3d"):
+        if args.workload == "batch4096":
+            n_pts, dims = 16384 * 4096, None
+            mk = lambda: jt.FFTBatch1D(4096, 16384, precision="f32")
+            mk64 = lambda: jt.FFTBatch1D(4096, 16384, precision="f64")
+            flops = 5.0 * n_pts * math.log2(4096)
+            desc = "16384 x 4096-point complexForward, FP32, standard normal"
+            base = torch.randn(2 * n_pts, dtype=torch.float32, device=dev, generator=g)
+        else:
+            dims = (512, 512, 256)
+            n_pts = 512 * 512 * 256
+            mk = lambda: jt.FloatFFT_3D(*dims)
+            mk64 = lambda: jt.DoubleFFT_3D(*dims)
+            flops = 5.0 * n_pts * math.log2(n_pts)
+            desc = "FloatFFT_3D 512x512x256 complexForward, uniform[-1,1)"
+            base = torch.rand(2 * n_pts, dtype=torch.float32, device=dev, generator=g) * 2 - 1
+        res = {}
+        ref = base.double()
+        mk64().complexForward(ref)
+        for form in ("plain", "split_fp16"):
+            plan = mk()
+            if form == "split_fp16":
+                plan.set_split_fp16(True)
+            bufs = [base.clone() for _ in range(W + K)]
+            calls = [(lambda b=b: plan.complexForward(b)) for b in bufs]
+            ms, prof = timed(torch, jt, calls, K, W)
+            chk = base.clone()
+            plan.complexForward(chk)
+            torch.cuda.synchronize()
+            res[form] = {"ms_per_step": ms, "GFLOPs": flops / (ms * 1e-3) / 1e9,
+                         "rel_l2_vs_fp64_gpu": rel(chk, ref), "roofline": roofline_from_profile(prof)}
+            del bufs
+        out.update({"config": desc, "dtype": "f32", "plain": res["plain"], "split_fp16": res["split_fp16"],
+                    "split_over_plain": res["plain"]["ms_per_step"] / res["split_fp16"]["ms_per_step"]})
+        return out
+
+    if args.workload in ("dct3d", "dst3d"):
+        dims = (512, 512, 512)
+        n_pts = 512 ** 3
+        plan = (jt.DoubleDCT_3D if args.workload == "dct3d" else jt.DoubleDST_3D)(*dims)
+        base = torch.rand(n_pts, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+        bufs = [base.clone() for _ in range(W + K)]
+        calls = [(lambda b=b: plan.forward(b, True)) for b in bufs]
+        ms, prof = timed(torch, jt, calls, K, W)
+        chk = base.clone()
+        plan.forward(chk, True)
+        plan.inverse(chk, True)
+        torch.cuda.synchronize()
+        flops = 2.5 * n_pts * math.log2(n_pts)
+        out.update({"config": f"{plan.__class__.__name__} 512^3 forward(scale=true), FP64", "dtype": "f64",
+                    "ms_per_step": ms, "GFLOPs": flops / (ms * 1e-3) / 1e9,
+                    "round_trip_rel_l2": rel(chk, base), "roofline": roofline_from_profile(prof),
+                    "step_GBps": 3 * 2 * n_pts * 8 / (ms * 1e-3) / 1e9})
+        return out
+
+    # fft1d: n=1024 FP64, batch 1: complexForward + complexInverse(scale) + realForward
+    n = 1024
+    plan = jt.DoubleFFT_1D(n)
+    base = torch.rand(2 * n, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    bufs = [base.clone() for _ in range(W + K)]
+
+    def step(b):
+        plan.complexForward(b)
+        plan.complexInverse(b, True)
+        plan.realForward(b)
+
+    calls = [(lambda b=b: step(b)) for b in bufs]
+    ms, prof = timed(torch, jt, calls, K, W)
+    out.update({"config": "DoubleFFT_1D(1024): complexForward + complexInverse(scale) + realForward, FP64",
+                "dtype": "f64", "us_per_step": ms * 1e3,
+                "kernel_us": {k: ms_ / c * 1e3 for k, (c, ms_) in prof.items()}})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +224,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="fft2d",
+                    choices=["fft2d", "batch4096", "fft3d", "dct3d", "dst3d", "fft1d"],
+                    help="fft2d (default): the headline 4096x4096 FP64 line; the others "
+                         "measure the remaining configurations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the transforms "
@@ -113,6 +237,10 @@ def main():
     import torch
 
     import jtransforms_amd as jt
+
+    if args.workload != "fft2d":
+        print(json.dumps(other_workload(args, torch, jt)))
+        return
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -148,7 +276,7 @@ def main():
     # time; its algorithmic bytes and duration are averaged over its launches
     fam = {}
     for k, (c, ms) in prof.items():
-        key = re.sub(r",lines=\d+", "", k)
+        key = re.sub(r",(lines|es)=\d+", "", k)
         f = fam.setdefault(key, [0, 0.0, 0.0])
         f[0] += c
         f[1] += ms

@@ -337,7 +337,7 @@ void go_tpl(IO io, const cpx<T>* tw, hipStream_t st) {
     if (groups <= 0) return;
     JT_PROF(st, std::string("stockham<") + (sizeof(T) == 8 ? "f64" : "f32") + ",N=" + std::to_string(N) +
  (COL ? ",col" : ",row") + (MF ? ",mfma" : "") + ",W=" + std::to_string(W) + ",lines=" + std::to_string(io.nlines) +
-                    "," + IO::tag() + ">");
+                    "," + IO::tag() + io.desc() + ">");
     hipLaunchKernelGGL((stockham_kernel<T, DIR, N, TL, W, COL, SPLIT, MF, IO, Rs...>), dim3((unsigned)groups),
                        dim3(TL * W), 0, st, io, tw);
     JT_HIP(hipGetLastError());
@@ -383,7 +383,7 @@ void go_gen(const Fft1D<T>& p, IO io, hipStream_t st) {
     }
     int64_t lines = io.nlines;
     JT_PROF(st, std::string("generic<") + (sizeof(T) == 8 ? "f64" : "f32") + ",N=" + std::to_string(n) +
-                    ",lines=" + std::to_string(lines) + "," + IO::tag() + ">");
+                    ",lines=" + std::to_string(lines) + "," + IO::tag() + io.desc() + ">");
     unsigned gx = (unsigned)(lines < 65535 ? lines : 65535);
     unsigned gy = (unsigned)((lines + gx - 1) / gx);
     hipLaunchKernelGGL((generic_kernel<T, DIR, IO>), dim3(gx, gy), dim3(nt), shm, st, io,

@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256) generic_kernel(IO io, const cpx<T>* __res
     const int64_t base = io.base(line);
 
     if constexpr (IO::pre_lds) {
-        io.pre(line, base, are, aim, t, nt);
+        io.template pre<0, 0>(line, base, are, aim, t, nt);
     } else {
         for (int k = t; k < n; k += nt) {
             cpx<T> x = io.load(base, k);
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256) generic_kernel(IO io, const cpx<T>* __res
     }
     __syncthreads();
     if constexpr (IO::post_lds) {
-        io.post(line, base, are, aim, t, nt);
+        io.template post<0, 0>(line, base, are, aim, t, nt);
     } else {
         for (int k = t; k < n; k += nt) io.store(base, k, cpx<T>{are[gpos(k)], aim[gpos(k)]});
     }

@@ -1,6 +1,8 @@
 // IO policies of the Stockham kernels: where a line's points come from, where
 // the results go, and the O(N) pre/post passes fused around the FFT.
 #pragma once
+#include <string>
+
 #include "jt_dft.hpp"
 
 namespace jt {
@@ -19,6 +21,7 @@ template <typename T>
 struct IOStrided {
     static constexpr bool pre_lds = false, post_lds = false;
     static const char* tag() { return "c2c"; }
+    std::string desc() const { return ",es=" + std::to_string(m.es); }
     const cpx<T>* in;
     cpx<T>* out;
     LineMap m;
@@ -30,7 +33,9 @@ struct IOStrided {
     __device__ __forceinline__ void store(int64_t b, int k, cpx<T> v) const {
         out[b + k * m.es] = cpx<T>{v.x * scale, v.y * scale};
     }
+    template <int CN, int CTL>
     __device__ void pre(int64_t, int64_t, T*, T*, int, int) const {}
+    template <int CN, int CTL>
     __device__ void post(int64_t, int64_t, const T*, const T*, int, int) const {}
 };
 
@@ -51,6 +56,7 @@ template <typename T, int DIR>
 struct IOFourA {
     static constexpr bool pre_lds = false, post_lds = false;
     static const char* tag() { return "fourA"; }
+    std::string desc() const { return ",n1=" + std::to_string(n1) + ",n2=" + std::to_string(n2); }
     const cpx<T>* in;
     cpx<T>* out;
     const cpx<T>* hi;
@@ -64,7 +70,9 @@ struct IOFourA {
         int64_t e = (j2 * (int64_t)k) % n;
         out[b + (int64_t)k * n2] = cmul(v, tw2<DIR>(hi, lo, e));
     }
+    template <int CN, int CTL>
     __device__ void pre(int64_t, int64_t, T*, T*, int, int) const {}
+    template <int CN, int CTL>
     __device__ void post(int64_t, int64_t, const T*, const T*, int, int) const {}
 };
 
@@ -74,6 +82,7 @@ template <typename T>
 struct IOFourB {
     static constexpr bool pre_lds = false, post_lds = false;
     static const char* tag() { return "fourB"; }
+    std::string desc() const { return ",n1=" + std::to_string(n1) + ",n2=" + std::to_string(n2); }
     const cpx<T>* in;
     cpx<T>* out;
     int64_t nlines, n, n1, n2;
@@ -86,7 +95,9 @@ struct IOFourB {
         int64_t bb = b / n, k1 = (b % n) / n2;
         out[bb * n + k1 + (int64_t)k * n1] = cpx<T>{v.x * scale, v.y * scale};
     }
+    template <int CN, int CTL>
     __device__ void pre(int64_t, int64_t, T*, T*, int, int) const {}
+    template <int CN, int CTL>
     __device__ void post(int64_t, int64_t, const T*, const T*, int, int) const {}
 };
 
@@ -100,6 +111,7 @@ template <typename T>
 struct IOR2C {
     static constexpr bool pre_lds = false, post_lds = true;
     static const char* tag() { return "r2c"; }
+    std::string desc() const { return ",es=" + std::to_string(m.es); }
     const cpx<T>* in;  // reals viewed as complex pairs
     T* out;            // packed reals
     const cpx<T>* rtw;
@@ -111,7 +123,9 @@ struct IOR2C {
     __device__ __forceinline__ int64_t base(int64_t l) const { return m.base(l); }
     __device__ __forceinline__ cpx<T> load(int64_t b, int k) const { return in[b + k]; }
     __device__ void store(int64_t, int, cpx<T>) const {}
+    template <int CN, int CTL>
     __device__ void pre(int64_t, int64_t, T*, T*, int, int) const {}
+    template <int CN, int CTL>
     __device__ void post(int64_t, int64_t b, const T* re, const T* im, int t, int nt) const {
         T* o = out + 2 * b;
         const T h = T(0.5) * scale;
@@ -149,6 +163,7 @@ template <typename T>
 struct IOC2R {
     static constexpr bool pre_lds = true, post_lds = false;
     static const char* tag() { return "c2r"; }
+    std::string desc() const { return ",es=" + std::to_string(m.es); }
     const T* in;     // packed reals
     cpx<T>* out;     // reals viewed as complex pairs
     const cpx<T>* rtw;
@@ -162,34 +177,55 @@ struct IOC2R {
     __device__ __forceinline__ void store(int64_t b, int k, cpx<T> v) const {
         out[b + k] = cpx<T>{v.x * scale, v.y * scale};
     }
-    __device__ void pre(int64_t, int64_t b, T* re, T* im, int t, int nt) const {
-        const T* a = in + 2 * b;
-        for (int k = t; k <= N / 2; k += nt) {
-            if (k == 0) {
-                T x0 = a[0], xn = a[1];
-                // E_0 = X_0 + X_N, O_0 = X_0 - X_N
-                re[lpos(0)] = x0 + xn;
-                im[lpos(0)] = x0 - xn;
-                continue;
-            }
-            int kk = N - k;
-            cpx<T> Xk = {a[2 * k], a[2 * k + 1]};
-            cpx<T> Xc = {a[2 * kk], -a[2 * kk + 1]};  // conj X_{N-k}
-            cpx<T> E = cadd(Xk, Xc);
-            cpx<T> D = csub(Xk, Xc);
-            cpx<T> w = {rtw[k].x, -rtw[k].y};  // exp(+2 pi i k/2N)
-            cpx<T> O = cmul(D, w);
-            // Z_k = E + i O
-            re[lpos(k)] = E.x - O.y;
-            im[lpos(k)] = E.y + O.x;
-            if (kk != k) {
-                // E_{N-k} = conj E_k and O_{N-k} = conj O_k, so
-                // Z_{N-k} = conj E + i conj O
-                re[lpos(kk)] = E.x + O.y;
-                im[lpos(kk)] = O.x - E.y;
-            }
+    // Z_k and Z_{N-k} from X_k, X_{N-k} (k in (0, N/2]); Z_0 from X_0, X_N
+    __device__ __forceinline__ void c2r_pair(int k, cpx<T> Xk, cpx<T> Xn, T* re, T* im) const {
+        int kk = N - k;
+        cpx<T> Xc = {Xn.x, -Xn.y};  // conj X_{N-k}
+        cpx<T> E = cadd(Xk, Xc);
+        cpx<T> D = csub(Xk, Xc);
+        cpx<T> w = {rtw[k].x, -rtw[k].y};  // exp(+2 pi i k/2N)
+        cpx<T> O = cmul(D, w);
+        // Z_k = E + i O
+        re[lpos(k)] = E.x - O.y;
+        im[lpos(k)] = E.y + O.x;
+        if (kk != k) {
+            // E_{N-k} = conj E_k and O_{N-k} = conj O_k, so Z_{N-k} = conj E + i conj O
+            re[lpos(kk)] = E.x + O.y;
+            im[lpos(kk)] = O.x - E.y;
         }
     }
+    template <int CN, int CTL>
+    __device__ void pre(int64_t, int64_t b, T* re, T* im, int t, int nt) const {
+        const T* a = in + 2 * b;
+        if (t == 0) {
+            T x0 = a[0], xn = a[1];
+            // E_0 = X_0 + X_N, O_0 = X_0 - X_N
+            re[lpos(0)] = x0 + xn;
+            im[lpos(0)] = x0 - xn;
+        }
+        if constexpr (CN > 0) {
+            // all loads first (compile-time trip count), then the LDS writes
+            constexpr int IT = (CN / 2 + CTL - 1) / CTL;
+            cpx<T> xk[IT], xn[IT];
+#pragma unroll
+            for (int q = 0; q < IT; q++) {
+                int k = 1 + t + q * CTL;
+                if (k <= CN / 2) {
+                    xk[q] = cpx<T>{a[2 * k], a[2 * k + 1]};
+                    xn[q] = cpx<T>{a[2 * (CN - k)], a[2 * (CN - k) + 1]};
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < IT; q++) {
+                int k = 1 + t + q * CTL;
+                if (k <= CN / 2) c2r_pair(k, xk[q], xn[q], re, im);
+            }
+        } else {
+            for (int k = 1 + t; k <= N / 2; k += nt)
+                c2r_pair(k, cpx<T>{a[2 * k], a[2 * k + 1]}, cpx<T>{a[2 * (N - k)], a[2 * (N - k) + 1]}, re, im);
+        }
+    }
+    template <int CN, int CTL>
     __device__ void post(int64_t, int64_t, const T*, const T*, int, int) const {}
 };
 

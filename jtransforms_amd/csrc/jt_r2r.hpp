@@ -56,6 +56,7 @@ struct IOR2R {
     static const char* tag() {
         return KIND == R2R_DHT ? "dht" : (KIND == R2R_DCT ? (FWD ? "dct2" : "dct3") : (FWD ? "dst2" : "dst3"));
     }
+    std::string desc() const { return ",es=" + std::to_string(m.es); }
     T* data;
     const cpx<T>* q;
     const cpx<T>* rtw;
@@ -84,50 +85,80 @@ struct IOR2R {
         return k == 0 ? v * f0 : v;
     }
 
+    // forward: element i of the line -> its slot in the N complex inputs
+    __device__ __forceinline__ void put_fwd(int i, T v, T* re, T* im) const {
+        int j;
+        if constexpr (KIND == R2R_DHT) j = i;  // natural pairs z_m = x_2m + i x_2m+1
+        else j = (i & 1) ? n - 1 - (i >> 1) : (i >> 1);  // Makhoul: v_j = x_2j, v_{n-1-j} = x_2j+1
+        int mm = j >> 1;
+        if (j & 1) im[lpos(mm)] = v;
+        else re[lpos(mm)] = v;
+    }
+    __device__ __forceinline__ T fwd_in(int64_t b, int i) const {
+        if constexpr (KIND == R2R_DHT) return rd(b, i);
+        else return xin(b, i);
+    }
+    // inverse: Z_k, Z_{N-k} from a'_k, a'_{n-k}, a'_{N-k}, a'_{N+k}
+    __device__ __forceinline__ void inv_pair(int k, T ak, T ank, T akk, T ankk, T* re, T* im) const {
+        int kk = N - k;
+        cpx<T> Vk = cmulc(cpx<T>{ak, -ank}, q[k]);
+        cpx<T> Vc = cmulc(cpx<T>{akk, -ankk}, q[kk]);
+        cpx<T> Xc = {Vc.x, -Vc.y};  // conj V_{N-k}
+        cpx<T> E = cadd(Vk, Xc), D = csub(Vk, Xc);
+        cpx<T> w = {rtw[k].x, -rtw[k].y};
+        cpx<T> O = cmul(D, w);
+        re[lpos(k)] = E.x - O.y;
+        im[lpos(k)] = E.y + O.x;
+        if (kk != k) {
+            re[lpos(kk)] = E.x + O.y;
+            im[lpos(kk)] = O.x - E.y;
+        }
+    }
+
+    template <int CN, int CTL>
     __device__ void pre(int64_t, int64_t b, T* re, T* im, int t, int nt) const {
         if constexpr (FWD || KIND == R2R_DHT) {
-            if constexpr (KIND == R2R_DHT) {
-                // natural pairs z_m = x_2m + i x_2m+1
-                for (int i = t; i < n; i += nt) {
-                    T v = rd(b, i);
-                    int mm = i >> 1;
-                    if (i & 1) im[lpos(mm)] = v;
-                    else re[lpos(mm)] = v;
-                }
+            if constexpr (CN > 0) {
+                // 2N reals per line: issue every load before the LDS writes
+                constexpr int IT = (2 * CN) / CTL;
+                T vals[IT];
+#pragma unroll
+                for (int q2 = 0; q2 < IT; q2++) vals[q2] = fwd_in(b, t + q2 * CTL);
+#pragma unroll
+                for (int q2 = 0; q2 < IT; q2++) put_fwd(t + q2 * CTL, vals[q2], re, im);
             } else {
-                // Makhoul permutation v_j = x_2j, v_{n-1-j} = x_{2j+1}; z_m = v_2m + i v_2m+1
-                for (int i = t; i < n; i += nt) {
-                    T v = xin(b, i);
-                    int j = (i & 1) ? n - 1 - (i >> 1) : (i >> 1);
-                    int mm = j >> 1;
-                    if (j & 1) im[lpos(mm)] = v;
-                    else re[lpos(mm)] = v;
-                }
+                for (int i = t; i < n; i += nt) put_fwd(i, fwd_in(b, i), re, im);
             }
         } else {
             // DCT-III / DST-III: V_k = conj(q_k)(a'_k - i a'_{n-k}), then the
             // real-inverse split into Z (N points)
-            for (int k = t; k <= N / 2; k += nt) {
-                if (k == 0) {
-                    T v0 = ain(b, 0);
-                    T vN = T(1.41421356237309504880168872420970) * ain(b, N);
-                    re[lpos(0)] = v0 + vN;
-                    im[lpos(0)] = v0 - vN;
-                    continue;
+            if (t == 0) {
+                T v0 = ain(b, 0);
+                T vN = T(1.41421356237309504880168872420970) * ain(b, N);
+                re[lpos(0)] = v0 + vN;
+                im[lpos(0)] = v0 - vN;
+            }
+            if constexpr (CN > 0) {
+                constexpr int IT = (CN / 2 + CTL - 1) / CTL;
+                T a0[IT], a1[IT], a2[IT], a3[IT];
+#pragma unroll
+                for (int q2 = 0; q2 < IT; q2++) {
+                    int k = 1 + t + q2 * CTL;
+                    if (k <= CN / 2) {
+                        a0[q2] = ain(b, k);
+                        a1[q2] = ain(b, n - k);
+                        a2[q2] = ain(b, N - k);
+                        a3[q2] = ain(b, N + k);
+                    }
                 }
-                int kk = N - k;
-                cpx<T> Vk = cmulc(cpx<T>{ain(b, k), -ain(b, n - k)}, q[k]);
-                cpx<T> Vc = cmulc(cpx<T>{ain(b, kk), -ain(b, n - kk)}, q[kk]);
-                cpx<T> Xc = {Vc.x, -Vc.y};  // conj V_{N-k}
-                cpx<T> E = cadd(Vk, Xc), D = csub(Vk, Xc);
-                cpx<T> w = {rtw[k].x, -rtw[k].y};
-                cpx<T> O = cmul(D, w);
-                re[lpos(k)] = E.x - O.y;
-                im[lpos(k)] = E.y + O.x;
-                if (kk != k) {
-                    re[lpos(kk)] = E.x + O.y;
-                    im[lpos(kk)] = O.x - E.y;
+#pragma unroll
+                for (int q2 = 0; q2 < IT; q2++) {
+                    int k = 1 + t + q2 * CTL;
+                    if (k <= CN / 2) inv_pair(k, a0[q2], a1[q2], a2[q2], a3[q2], re, im);
                 }
+            } else {
+                for (int k = 1 + t; k <= N / 2; k += nt)
+                    inv_pair(k, ain(b, k), ain(b, n - k), ain(b, N - k), ain(b, N + k), re, im);
             }
         }
     }
@@ -140,6 +171,7 @@ struct IOR2R {
         else wr(b, k, v * s);
     }
 
+    template <int CN, int CTL>
     __device__ void post(int64_t, int64_t b, const T* re, const T* im, int t, int nt) const {
         if constexpr (FWD || KIND == R2R_DHT) {
             // real-FFT split V_k / V_{N-k} from Z, then the transform outputs

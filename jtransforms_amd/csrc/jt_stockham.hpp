@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(TL* W) stockham_kernel(IO io, const cpx<T>* __
 
     cpx<T> v[E];
     if constexpr (IO::pre_lds) {
-        if (active) io.pre(line, base, lds_re + w * G::pitch, lds_im + w * G::pitch, t, TL);
+        if (active) io.template pre<N, TL>(line, base, lds_re + w * G::pitch, lds_im + w * G::pitch, t, TL);
         __syncthreads();
         Eng::template get<R0, 0>(lds_re, w, t, v);
         Eng::template get<R0, 1>(lds_im, w, t, v);
@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(TL* W) stockham_kernel(IO io, const cpx<T>* __
 
     if constexpr (IO::post_lds) {
         Eng::template put_final<RL, NL>(lds, w, t, v);
-        if (active) io.post(line, base, lds_re + w * G::pitch, lds_im + w * G::pitch, t, TL);
+        if (active) io.template post<N, TL>(line, base, lds_re + w * G::pitch, lds_im + w * G::pitch, t, TL);
     } else {
         if (active) {
 #pragma unroll
