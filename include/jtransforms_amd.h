@@ -53,7 +53,10 @@ enum {
     JT_FLAG_NONE = 0,
     /* radix-16 stages as split-FP16 MFMA products with FP32 accumulation
      * (FP32 plans only; no counterpart in the reference) */
-    JT_FLAG_SPLIT_FP16 = 1
+    JT_FLAG_SPLIT_FP16 = 1,
+    /* ragged batches: capture the launch sequence in a hipGraph on the first
+     * call for a given (data, stream, op) and replay it afterwards */
+    JT_FLAG_GRAPH = 2
 };
 
 /* operations (one per reference method family) */
@@ -101,6 +104,19 @@ int jt_plan_create_dht(jt_plan** out, int precision, int ndim, const int64_t* di
  * fft/DoubleFFT_2D.java:3100-3140). */
 int jt_plan_create_fft1d_batch(jt_plan** out, int precision, int64_t n, int64_t batch,
                                uint32_t flags);
+
+/* `count` independent 1D complex transforms of the given lengths, stored
+ * back to back (transform i starts at element 2*sum_{j<i} lengths[j]), run in
+ * one persistent launch (JT_COMPLEX_FORWARD / JT_COMPLEX_INVERSE, scale = 1/n
+ * per transform).  No counterpart in the reference, whose callers build one
+ * DoubleFFT_1D per length (fft/DoubleFFT_1D.java:117) and call it per
+ * transform. */
+int jt_plan_create_ragged(jt_plan** out, int precision, int64_t count, const int64_t* lengths,
+                          uint32_t flags);
+
+/* Waits for the device and reports a failure recorded on it by the plan's
+ * last launches (a ragged batch whose dependency wait timed out). */
+int jt_plan_sync_status(jt_plan* plan);
 
 /* Sets plan flags after creation (JT_FLAG_SPLIT_FP16 on FP32 FFT plans of
  * any rank, e.g. a FloatFFT_3D).  No counterpart in the reference: selects

@@ -22,7 +22,7 @@ __all__ = [
     "DoubleDCT_1D", "FloatDCT_1D", "DoubleDCT_2D", "FloatDCT_2D", "DoubleDCT_3D", "FloatDCT_3D",
     "DoubleDST_1D", "FloatDST_1D", "DoubleDST_2D", "FloatDST_2D", "DoubleDST_3D", "FloatDST_3D",
     "DoubleDHT_1D", "FloatDHT_1D", "DoubleDHT_2D", "FloatDHT_2D", "DoubleDHT_3D", "FloatDHT_3D",
-    "FFTBatch1D", "IllegalArgumentException", "JTransformsError", "lib", "LIB_PATH",
+    "FFTBatch1D", "RaggedBatch1D", "IllegalArgumentException", "JTransformsError", "lib", "LIB_PATH",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -32,6 +32,7 @@ LIB_PATH = os.path.join(_HERE, "libjtamd.so")
 JT_OK, JT_BAD_SIZE, JT_NOT_POW2, JT_TOO_BIG, JT_HIP_ERROR, JT_OOM, JT_BAD_ARG, JT_UNSUPPORTED = range(8)
 JT_F64, JT_F32 = 0, 1
 JT_FLAG_SPLIT_FP16 = 1
+JT_FLAG_GRAPH = 2
 (COMPLEX_FORWARD, COMPLEX_INVERSE, REAL_FORWARD, REAL_INVERSE, REAL_FORWARD_FULL,
  REAL_INVERSE_FULL, FORWARD, INVERSE) = range(8)
 
@@ -40,6 +41,7 @@ EXPORTED_SYMBOLS = [
     "jt_plan_create_dst", "jt_plan_create_dht", "jt_plan_create_fft1d_batch", "jt_plan_destroy",
     "jt_execute", "jt_execute_host", "jt_plan_elements", "jt_status_string", "jt_last_error",
     "jt_version", "jt_profile_begin", "jt_profile_end", "jt_plan_set_flags",
+    "jt_plan_create_ragged", "jt_plan_sync_status",
 ]
 
 
@@ -80,6 +82,8 @@ def lib():
         L.jt_plan_create_fft1d_batch.argtypes = [P, ctypes.c_int, i64, i64, ctypes.c_uint32]
         L.jt_plan_destroy.argtypes = [ctypes.c_void_p]
         L.jt_plan_set_flags.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+        L.jt_plan_create_ragged.argtypes = [P, ctypes.c_int, i64, ctypes.POINTER(i64), ctypes.c_uint32]
+        L.jt_plan_sync_status.argtypes = [ctypes.c_void_p]
         L.jt_execute.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, i64, ctypes.c_int,
                                  ctypes.c_void_p]
         L.jt_execute_host.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, i64,
@@ -258,6 +262,31 @@ class FFTBatch1D(_FFTMethods):
         self.n, self.batch = int(n), int(batch)
         flags = JT_FLAG_SPLIT_FP16 if split_fp16 else 0
         super().__init__(lib().jt_plan_create_fft1d_batch, self.n, self.batch, flags)
+
+
+class RaggedBatch1D(_Plan):
+    """Many 1D complex transforms of unequal lengths stored back to back, run
+    in one persistent launch (no reference counterpart: it replaces one
+    DoubleFFT_1D per length).  graph=True replays a captured hipGraph."""
+
+    def __init__(self, lengths, precision: str = "f64", graph: bool = False):
+        self._precision = JT_F64 if precision in ("f64", "double") else JT_F32
+        self.lengths = np.ascontiguousarray(lengths, dtype=np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(2 * self.lengths)])
+        arr = self.lengths.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        super().__init__(lib().jt_plan_create_ragged, len(self.lengths), arr,
+                         JT_FLAG_GRAPH if graph else 0)
+
+    def complexForward(self, a):
+        self._run(COMPLEX_FORWARD, a, 0)
+
+    def complexInverse(self, a, scale: bool):
+        self._run(COMPLEX_INVERSE, a, 0, scale)
+
+    def sync_status(self):
+        rc = lib().jt_plan_sync_status(self._h)
+        if rc:
+            _raise(rc)
 
 
 class _R2R(_Plan):

@@ -7,6 +7,7 @@
 #include "../../include/jtransforms_amd.h"
 #include "jt_multid.hpp"
 #include "jt_r2r.hpp"
+#include "jt_ragged.hpp"
 
 using namespace jt;
 
@@ -14,7 +15,7 @@ namespace {
 
 thread_local std::string g_err;
 
-enum Family { FAM_FFT = 0, FAM_DCT = 1, FAM_DST = 2, FAM_DHT = 3 };
+enum Family { FAM_FFT = 0, FAM_DCT = 1, FAM_DST = 2, FAM_DHT = 3, FAM_RAGGED = 4 };
 
 template <typename T>
 struct Impl {
@@ -22,6 +23,7 @@ struct Impl {
     std::unique_ptr<Fft1D<T>> ax[3];
     std::unique_ptr<Real1D<T>> real;  // along the last axis
     std::unique_ptr<R2R1D<T>> r2r[3];
+    std::unique_ptr<RaggedPlan<T>> ragged;
     DevBuf ws;       // workspace
     DevBuf staging;  // host-array execution
 };
@@ -274,6 +276,13 @@ template <typename T>
 int exec_t(jt_plan* p, int op, T* a, bool scale, hipStream_t st) {
     Impl<T>& I = impl<T>(p);
     cpx<T>* ws = I.ws.template as<cpx<T>>();
+    if (p->family == FAM_RAGGED) {
+        if (op != JT_COMPLEX_FORWARD && op != JT_COMPLEX_INVERSE)
+            return set_err(JT_BAD_ARG, "a ragged batch supports complexForward / complexInverse");
+        ragged_execute<T>(*I.ragged, reinterpret_cast<cpx<T>*>(a), op == JT_COMPLEX_FORWARD ? -1 : 1, scale, st,
+                          (p->flags & JT_FLAG_GRAPH) != 0);
+        return JT_OK;
+    }
     if (p->family != FAM_FFT) {
         if (op != JT_FORWARD && op != JT_INVERSE) return set_err(JT_BAD_ARG, "op not defined for this plan");
         run_r2r_nd<T>(p->family, p->ndim, p->dims, I.r2r, a, op == JT_FORWARD, scale, ws, st);
@@ -371,6 +380,47 @@ int jt_plan_create_fft1d_batch(jt_plan** out, int precision, int64_t n, int64_t 
     return create(out, FAM_FFT, precision, 1, &n, batch, flags);
 }
 
+int jt_plan_create_ragged(jt_plan** out, int precision, int64_t count, const int64_t* lengths, uint32_t flags) {
+    return guarded([&]() -> int {
+        if (!out) return set_err(JT_BAD_ARG, "null plan pointer");
+        *out = nullptr;
+        if (precision != JT_F64 && precision != JT_F32) return set_err(JT_BAD_ARG, "unknown precision");
+        if (count < 1 || !lengths) return set_err(JT_BAD_SIZE, "count must be greater than 0");
+        if (flags & ~(uint32_t)JT_FLAG_GRAPH) return set_err(JT_BAD_ARG, "unknown flags");
+        for (int64_t i = 0; i < count; i++)
+            if (lengths[i] < 1) return set_err(JT_BAD_SIZE, "n must be greater than 0");
+        auto p = std::make_unique<jt_plan>();
+        p->family = FAM_RAGGED;
+        p->precision = precision;
+        p->ndim = 1;
+        p->dims[0] = count;
+        p->flags = flags;
+        if (precision == JT_F64) {
+            p->d = std::make_unique<Impl<double>>();
+            p->d->ragged = make_ragged<double>(lengths, count);
+        } else {
+            p->f = std::make_unique<Impl<float>>();
+            p->f->ragged = make_ragged<float>(lengths, count);
+        }
+        *out = p.release();
+        return JT_OK;
+    });
+}
+
+int jt_plan_sync_status(jt_plan* p) {
+    return guarded([&]() -> int {
+        if (!p) return set_err(JT_BAD_ARG, "null plan");
+        JT_HIP(hipDeviceSynchronize());
+        if (p->family == FAM_RAGGED) {
+            int e = 0;
+            DevBuf& d = p->precision == JT_F64 ? p->d->ragged->d_err : p->f->ragged->d_err;
+            JT_HIP(hipMemcpy(&e, d.p, sizeof(int), hipMemcpyDeviceToHost));
+            if (e) return set_err(JT_HIP_ERROR, "a ragged-batch wait timed out; results are incomplete");
+        }
+        return JT_OK;
+    });
+}
+
 int jt_plan_set_flags(jt_plan* p, uint32_t flags) {
     return guarded([&]() -> int {
         if (!p) return set_err(JT_BAD_ARG, "null plan");
@@ -378,6 +428,7 @@ int jt_plan_set_flags(jt_plan* p, uint32_t flags) {
         if ((flags & JT_FLAG_SPLIT_FP16) && p->precision != JT_F32)
             return set_err(JT_BAD_ARG, "split-FP16 form is defined for FP32 plans only");
         if (p->family != FAM_FFT) return set_err(JT_BAD_ARG, "flags apply to FFT plans");
+        if (!p->f) return set_err(JT_BAD_ARG, "split-FP16 form is defined for FP32 plans only");
         p->flags = flags;
         apply_flags(*p->f, flags);
         return JT_OK;
@@ -393,6 +444,10 @@ int jt_plan_destroy(jt_plan* plan) {
 
 int64_t jt_plan_elements(const jt_plan* p, int op) {
     if (!p) return -1;
+    if (p->family == FAM_RAGGED) {
+        int64_t t = p->precision == JT_F64 ? p->d->ragged->total : p->f->ragged->total;
+        return (op == JT_COMPLEX_FORWARD || op == JT_COMPLEX_INVERSE) ? 2 * t : -1;
+    }
     const int64_t tot = total(p) * p->batch;
     if (p->family != FAM_FFT) return tot;
     switch (op) {
@@ -434,7 +489,8 @@ int jt_execute_host(jt_plan* p, int op, void* data, int64_t offset, int scale) {
         JT_HIP(hipMemcpy(stg.p, h, bytes, hipMemcpyHostToDevice));
         int rc = jt_execute(p, op, stg.p, 0, scale, nullptr);
         if (rc != JT_OK) return rc;
-        JT_HIP(hipDeviceSynchronize());
+        rc = jt_plan_sync_status(p);
+        if (rc != JT_OK) return rc;
         JT_HIP(hipMemcpy(h, stg.p, bytes, hipMemcpyDeviceToHost));
         return JT_OK;
     });

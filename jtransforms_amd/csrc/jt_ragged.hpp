@@ -1,0 +1,626 @@
+// Ragged batches: many 1D complex transforms of unequal lengths, stored back
+// to back, in ONE persistent launch.
+//
+// The reference transforms such a set with one DoubleFFT_1D per length, each
+// picking SPLIT_RADIX / MIXED_RADIX / BLUESTEIN (DoubleFFT_1D.java:117-188).
+// Here every length gets a per-length descriptor (radix plan, four-step split
+// or Bluestein convolution length) and the whole set becomes a queue of work
+// items executed by a persistent kernel:
+//   SINGLE     a length that fits one workgroup's LDS
+//   FOUR_A/B   the column / row halves of a four-step length (n > LDS limit)
+//   BLUE_S     Bluestein with its convolution length m in LDS (all in one item)
+//   BLUE_1..4  Bluestein with a four-step m: premul+columns, rows+spectrum
+//              product, inverse columns, inverse rows+postmul
+// Workgroups take items in queue order with an atomic counter.  An item that
+// consumes another phase's output waits on a per-transform counter; every
+// producer of that phase sits earlier in the queue and was therefore taken by
+// a running workgroup before the consumer, so the wait always ends.  Each wait
+// is also bounded in time (an error flag records a timeout and the item is
+// abandoned) so the grid always drains.  Counters are cleared by a kernel, not
+// a memset, so the pair of launches can be captured in a hipGraph.
+#pragma once
+#include <functional>
+#include <map>
+
+#include "jt_generic.hpp"
+#include "jt_fft1d.hpp"
+
+namespace jt {
+
+enum RKind { RK_SINGLE = 0, RK_FOUR = 1, RK_BLUE_S = 2, RK_BLUE_L = 3 };
+enum RType { RT_SINGLE = 0, RT_FOUR_A = 1, RT_FOUR_B = 2, RT_BLUE_S = 3, RT_BLUE_1 = 4, RT_BLUE_2 = 5,
+             RT_BLUE_3 = 6, RT_BLUE_4 = 7 };
+
+struct RLen {
+    int n;
+    int kind;
+    int nst;
+    int radix[GEN_MAX_STAGES];
+    int twoff[GEN_MAX_STAGES];  // offsets into the shared table (complex units)
+    int n1, n2;                 // four-step split (of n, or of m for Bluestein)
+    int l1, l2;                 // descriptor indices of the n1 / n2 lengths
+    int64_t hioff, looff;       // four-step twiddles exp(-2 pi i e / N): hi[e>>8] lo[e&255]
+    int m, lm;                  // Bluestein convolution length and its descriptor
+    int64_t choff, bsoff;       // chirp (n entries), kernel spectrum (m entries)
+};
+
+struct RItem {
+    int type;
+    int len;     // descriptor index
+    int c0, cnt; // first column/row and count (four-step items)
+    int64_t tr;  // transform index
+};
+
+template <typename T>
+struct RArgs {
+    cpx<T>* data;
+    const int64_t* off;   // complex offset of each transform
+    const int64_t* wsoff; // workspace offset of each transform (multi-phase ones)
+    cpx<T>* ws;
+    const RLen* lens;
+    const cpx<T>* tw;
+    const RItem* items;
+    int nitems;
+    int* next;     // queue head
+    int* cnt;      // 3 phase counters per transform
+    int* err;      // set to 1 if a wait timed out
+    int dir;       // -1 forward, +1 backward
+    int scaled;    // multiply each transform by 1/n (scaled inverse)
+    int64_t ntr;
+};
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+
+// FFT of one line held in LDS (a buffers, gpos layout), runtime radices;
+// returns 0 if the result is in a, 1 if in b.
+template <typename T>
+__device__ int rline_fft(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T* are, T* aim, T* bre,
+                         T* bim, int t, int nt) {
+    int ns = 1;
+    int which = 0;
+    for (int s = 0; s < L.nst; s++) {
+        __syncthreads();
+        const int R = L.radix[s];
+        const T* sr = which ? bre : are;
+        const T* si = which ? bim : aim;
+        T* dr = which ? are : bre;
+        T* di = which ? aim : bim;
+        if (dir < 0) gen_stage_rt<-1>(R, sr, si, dr, di, tw + L.twoff[s], L.n, ns, t, nt);
+        else gen_stage_rt<1>(R, sr, si, dr, di, tw + L.twoff[s], L.n, ns, t, nt);
+        ns *= R;
+        which ^= 1;
+    }
+    __syncthreads();
+    return which;
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> rtw2(int dir, const cpx<T>* __restrict__ tw, const RLen& L, int64_t e,
+                                       int64_t N) {
+    e %= N;
+    cpx<T> w = cmul(tw[L.hioff + (e >> 8)], tw[L.looff + (e & 255)]);
+    return dir < 0 ? w : cpx<T>{w.x, -w.y};
+}
+
+// Wait until *c >= target (thread 0 polls, the workgroup then proceeds).
+// Bounded: after ~2 s of wall clock the wait gives up and flags an error.
+__device__ bool rwait(int* c, int target, int* err) {
+    __shared__ int ok;
+    if (threadIdx.x == 0) {
+        int good = 1;
+        uint64_t t0 = wall_clock64();
+        while (__hip_atomic_load(c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(8);
+            if (wall_clock64() - t0 > 200000000ull) {  // 100 MHz counter: 2 s
+                good = 0;
+                atomicExch(err, 1);
+                break;
+            }
+        }
+        ok = good;
+    }
+    __syncthreads();
+    __threadfence();
+    return ok != 0;
+}
+
+__device__ __forceinline__ void rsignal(int* c, int add) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(c, add, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__global__ void ragged_reset(RArgs<T> a) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    for (; i < 3 * a.ntr; i += (int64_t)gridDim.x * blockDim.x) a.cnt[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *a.next = 0;
+        *a.err = 0;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) ragged_kernel(RArgs<T> a, int np) {
+    extern __shared__ unsigned char smem_raw[];
+    T* sm = reinterpret_cast<T*>(smem_raw);
+    T* are = sm;
+    T* aim = sm + np;
+    T* bre = sm + 2 * np;
+    T* bim = sm + 3 * np;
+    __shared__ int cur;
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int dir = a.dir;
+    for (;;) {
+        __syncthreads();
+        if (t == 0) cur = atomicAdd(a.next, 1);
+        __syncthreads();
+        const int it = cur;
+        if (it >= a.nitems) break;  // uniform across the workgroup
+        const RItem item = a.items[it];
+        const RLen& L = a.lens[item.len];
+        const int64_t tr = item.tr;
+        const T scale = a.scaled ? T(1) / T(L.n) : T(1);
+        cpx<T>* x = a.data + a.off[tr];
+        int* c3 = a.cnt + 3 * tr;
+        switch (item.type) {
+            case RT_SINGLE: {
+                const int n = L.n;
+                for (int k = t; k < n; k += nt) {
+                    cpx<T> v = x[k];
+                    are[gpos(k)] = v.x;
+                    aim[gpos(k)] = v.y;
+                }
+                int w = rline_fft<T>(dir, L, a.tw, are, aim, bre, bim, t, nt);
+                const T* rr = w ? bre : are;
+                const T* ri = w ? bim : aim;
+                for (int k = t; k < n; k += nt) x[k] = cpx<T>{rr[gpos(k)] * scale, ri[gpos(k)] * scale};
+                break;
+            }
+            case RT_FOUR_A:
+            case RT_BLUE_1:
+            case RT_BLUE_3: {
+                // columns j2 in [c0, c0+cnt) of the n1 x n2 view; FFT_n1 then
+                // the exp(d 2 pi i j2 k1 / N) twiddle, into the workspace
+                const bool blue = item.type != RT_FOUR_A;
+                const RLen& B = blue ? a.lens[L.lm] : L;  // the four-step length
+                const RLen& L1 = a.lens[B.l1];
+                const int n1 = B.n1, n2 = B.n2;
+                const int64_t N = (int64_t)n1 * n2;
+                const int d = item.type == RT_BLUE_1 ? -1 : (item.type == RT_BLUE_3 ? 1 : dir);
+                cpx<T>* wsA = a.ws + a.wsoff[tr];
+                const cpx<T>* src = item.type == RT_FOUR_A ? x : (item.type == RT_BLUE_1 ? x : wsA + N);
+                if (item.type == RT_BLUE_3 && !rwait(c3 + 1, B.n1, a.err)) break;
+                for (int j2 = item.c0; j2 < item.c0 + item.cnt; j2++) {
+                    __syncthreads();  // the previous line's results have been read
+                    for (int j1 = t; j1 < n1; j1 += nt) {
+                        int64_t e = (int64_t)j1 * n2 + j2;
+                        cpx<T> v;
+                        if (item.type == RT_BLUE_1) {
+                            v = cpx<T>{T(0), T(0)};
+                            if (e < L.n) {
+                                cpx<T> c = a.tw[L.choff + e];
+                                v = dir < 0 ? cmulc(src[e], c) : cmul(src[e], c);
+                            }
+                        } else {
+                            v = src[e];
+                        }
+                        are[gpos(j1)] = v.x;
+                        aim[gpos(j1)] = v.y;
+                    }
+                    int w = rline_fft<T>(d, L1, a.tw, are, aim, bre, bim, t, nt);
+                    const T* rr = w ? bre : are;
+                    const T* ri = w ? bim : aim;
+                    for (int k1 = t; k1 < n1; k1 += nt) {
+                        cpx<T> v = {rr[gpos(k1)], ri[gpos(k1)]};
+                        v = cmul(v, rtw2<T>(d, a.tw, B, (int64_t)j2 * k1, N));
+                        wsA[(int64_t)k1 * n2 + j2] = v;
+                    }
+                }
+                rsignal(item.type == RT_BLUE_3 ? c3 + 2 : c3 + 0, item.cnt);
+                break;
+            }
+            case RT_FOUR_B:
+            case RT_BLUE_2:
+            case RT_BLUE_4: {
+                // rows k1 in [c0, c0+cnt): FFT_n2 of the workspace row; output
+                // point k2 is element k1 + n1*k2
+                const bool blue = item.type != RT_FOUR_B;
+                const RLen& B = blue ? a.lens[L.lm] : L;
+                const RLen& L2 = a.lens[B.l2];
+                const int n1 = B.n1, n2 = B.n2;
+                const int64_t N = (int64_t)n1 * n2;
+                const int d = item.type == RT_BLUE_2 ? -1 : (item.type == RT_BLUE_4 ? 1 : dir);
+                cpx<T>* wsA = a.ws + a.wsoff[tr];
+                const int* wc = item.type == RT_BLUE_4 ? c3 + 2 : c3 + 0;
+                if (!rwait(const_cast<int*>(wc), n2, a.err)) break;
+                for (int k1 = item.c0; k1 < item.c0 + item.cnt; k1++) {
+                    __syncthreads();  // the previous line's results have been read
+                    for (int j2 = t; j2 < n2; j2 += nt) {
+                        cpx<T> v = wsA[(int64_t)k1 * n2 + j2];
+                        are[gpos(j2)] = v.x;
+                        aim[gpos(j2)] = v.y;
+                    }
+                    int w = rline_fft<T>(d, L2, a.tw, are, aim, bre, bim, t, nt);
+                    const T* rr = w ? bre : are;
+                    const T* ri = w ? bim : aim;
+                    for (int k2 = t; k2 < n2; k2 += nt) {
+                        int64_t K = k1 + (int64_t)n1 * k2;
+                        cpx<T> v = {rr[gpos(k2)], ri[gpos(k2)]};
+                        if (item.type == RT_FOUR_B) {
+                            x[K] = cpx<T>{v.x * scale, v.y * scale};
+                        } else if (item.type == RT_BLUE_2) {
+                            cpx<T> b = a.tw[L.bsoff + K];
+                            wsA[N + K] = dir < 0 ? cmul(v, b) : cmulc(v, b);
+                        } else if (K < L.n) {
+                            cpx<T> c = a.tw[L.choff + K];
+                            v = dir < 0 ? cmulc(v, c) : cmul(v, c);
+                            x[K] = cpx<T>{v.x * scale, v.y * scale};
+                        }
+                    }
+                }
+                if (item.type == RT_BLUE_2) rsignal(c3 + 1, item.cnt);
+                break;
+            }
+            case RT_BLUE_S: {
+                // premul, FFT_m, spectrum product, inverse FFT_m, postmul in LDS
+                const RLen& M = a.lens[L.lm];
+                const int n = L.n, m = M.n;
+                for (int k = t; k < m; k += nt) {
+                    cpx<T> v = {T(0), T(0)};
+                    if (k < n) {
+                        cpx<T> c = a.tw[L.choff + k];
+                        v = dir < 0 ? cmulc(x[k], c) : cmul(x[k], c);
+                    }
+                    are[gpos(k)] = v.x;
+                    aim[gpos(k)] = v.y;
+                }
+                int w = rline_fft<T>(-1, M, a.tw, are, aim, bre, bim, t, nt);
+                T* rr = w ? bre : are;
+                T* ri = w ? bim : aim;
+                T* qr = w ? are : bre;
+                T* qi = w ? aim : bim;
+                for (int k = t; k < m; k += nt) {
+                    cpx<T> v = {rr[gpos(k)], ri[gpos(k)]};
+                    cpx<T> b = a.tw[L.bsoff + k];
+                    v = dir < 0 ? cmul(v, b) : cmulc(v, b);
+                    rr[gpos(k)] = v.x;
+                    ri[gpos(k)] = v.y;
+                }
+                int w2 = rline_fft<T>(1, M, a.tw, rr, ri, qr, qi, t, nt);
+                const T* fr = w2 ? qr : rr;
+                const T* fi = w2 ? qi : ri;
+                for (int k = t; k < n; k += nt) {
+                    cpx<T> v = {fr[gpos(k)], fi[gpos(k)]};
+                    cpx<T> c = a.tw[L.choff + k];
+                    v = dir < 0 ? cmulc(v, c) : cmul(v, c);
+                    x[k] = cpx<T>{v.x * scale, v.y * scale};
+                }
+                break;
+            }
+            default:
+                break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host plan
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RaggedPlan {
+    int64_t ntr = 0, total = 0;  // transforms, total complex points
+    int maxn = 0;                // largest LDS line length
+    int nitems = 0;
+    std::vector<int64_t> lens_of;  // per transform
+    DevBuf d_lens, d_tw, d_items, d_off, d_wsoff, d_cnt, d_next, d_err, ws;
+    int grid = 0;
+    size_t shm = 0;
+    int np = 0;
+    // hipGraph cache: one executable graph per (data, stream, dir, scale)
+    struct GKey {
+        void* data;
+        hipStream_t st;
+        int dir;
+        bool scale;
+    };
+    std::vector<std::pair<GKey, hipGraphExec_t>> graphs;
+    ~RaggedPlan() {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    }
+};
+
+// four-step split n = n1*n2 with both factors <= lim and smooth, balanced
+inline bool ragged_split(int64_t n, int64_t lim, int64_t& n1, int64_t& n2) {
+    int64_t best = 0;
+    for (int64_t a = 2; a <= lim; a++) {
+        if (n % a) continue;
+        int64_t b = n / a;
+        if (b > lim) continue;
+        std::vector<int> ra, rb;
+        if (!factor_radices(a, ra) || !factor_radices(b, rb)) continue;
+        if (!best || std::llabs(a - b) < std::llabs(best - n / best)) best = a;
+    }
+    if (!best) return false;
+    n1 = best;
+    n2 = n / best;
+    return true;
+}
+
+// Kernel spectrum of Bluestein: FFT_m of the symmetric chirp, / m, on the
+// GPU in FP64 (the same engine), returned to the host.
+inline std::vector<cpx<double>> bluestein_spectrum(int64_t n, int64_t m, const std::vector<cpx<double>>& chirp) {
+    std::vector<cpx<double>> b(m, cpx<double>{0.0, 0.0});
+    for (int64_t j = 0; j < n; j++) {
+        b[j] = chirp[j];
+        if (j) b[m - j] = chirp[j];
+    }
+    auto p = make_fft1d<double>(m);
+    DevBuf d, w;
+    upload(d, b);
+    w.alloc((size_t)std::max<int64_t>(p->workspace(1), 1) * sizeof(cpx<double>));
+    c2c_contig<double, -1>(*p, d.as<cpx<double>>(), d.as<cpx<double>>(), 1, 1.0 / (double)m,
+                           w.as<cpx<double>>(), 0);
+    JT_HIP(hipDeviceSynchronize());
+    JT_HIP(hipMemcpy(b.data(), d.p, m * sizeof(cpx<double>), hipMemcpyDeviceToHost));
+    return b;
+}
+
+template <typename T>
+std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count) {
+    auto P = std::make_unique<RaggedPlan<T>>();
+    P->ntr = count;
+    // longest line kept in one workgroup's LDS: two ping-pong buffers of this
+    // length take ~67 KiB, so two persistent workgroups fit per CU
+    const int64_t lim = sizeof(T) == 8 ? 2048 : 4096;
+    std::vector<RLen> lens;
+    std::vector<cpx<T>> tab;
+    std::map<int64_t, int> idx_single;  // single-kernel descriptor by n
+    std::map<int64_t, int> idx_len;     // top-level descriptor by n
+    auto add_twiddles = [&](const std::vector<int>& r, RLen& L) {
+        std::vector<int> offs;
+        auto tw = stage_twiddles<T>(r, &offs);
+        int64_t base = (int64_t)tab.size();
+        tab.insert(tab.end(), tw.begin(), tw.end());
+        L.nst = (int)r.size();
+        for (size_t i = 0; i < r.size(); i++) {
+            L.radix[i] = r[i];
+            L.twoff[i] = (int)(base + offs[i]);
+        }
+    };
+    std::function<int(int64_t)> single_of = [&](int64_t n) -> int {
+        auto it = idx_single.find(n);
+        if (it != idx_single.end()) return it->second;
+        RLen L{};
+        L.n = (int)n;
+        L.kind = RK_SINGLE;
+        std::vector<int> r;
+        if (!factor_radices(n, r)) throw JtError(ST_UNSUPPORTED, "internal: non-smooth single length");
+        add_twiddles(r, L);
+        lens.push_back(L);
+        P->maxn = std::max<int>(P->maxn, (int)n);
+        return idx_single[n] = (int)lens.size() - 1;
+    };
+    auto four_of = [&](int64_t n, RLen& L) {
+        int64_t n1, n2;
+        if (!ragged_split(n, lim, n1, n2)) throw JtError(ST_UNSUPPORTED, "length too large for the ragged plan");
+        L.n1 = (int)n1;
+        L.n2 = (int)n2;
+        int a1 = single_of(n1), a2 = single_of(n2);
+        L.l1 = a1;
+        L.l2 = a2;
+        L.hioff = (int64_t)tab.size();
+        for (int64_t i = 0; i <= (n >> 8); i++) tab.push_back(expm<T>(i << 8, n));
+        L.looff = (int64_t)tab.size();
+        for (int64_t i = 0; i < 256; i++) tab.push_back(expm<T>(i, n));
+    };
+    std::function<int(int64_t)> len_of = [&](int64_t n) -> int {
+        auto it = idx_len.find(n);
+        if (it != idx_len.end()) return it->second;
+        std::vector<int> r;
+        int id;
+        if (factor_radices(n, r) && n <= lim) {
+            id = single_of(n);
+        } else if (factor_radices(n, r)) {
+            RLen L{};
+            L.n = (int)n;
+            L.kind = RK_FOUR;
+            four_of(n, L);
+            lens.push_back(L);
+            id = (int)lens.size() - 1;
+        } else {
+            RLen L{};
+            L.n = (int)n;
+            int64_t m = next_smooth(2 * n - 1);
+            L.m = (int)m;
+            L.kind = m <= lim ? RK_BLUE_S : RK_BLUE_L;
+            // chirp exp(i pi (j^2 mod 2n)/n) by the integer recurrence
+            std::vector<cpx<double>> ch(n);
+            const long double PI = 3.141592653589793238462643383279502884L;
+            int64_t k = 0;
+            for (int64_t j = 0; j < n; j++) {
+                if (j) {
+                    k += 2 * j - 1;
+                    if (k >= 2 * n) k -= 2 * n;
+                }
+                long double ang = PI * (long double)k / (long double)n;
+                ch[j] = cpx<double>{(double)cosl(ang), (double)sinl(ang)};
+            }
+            auto bs = bluestein_spectrum(n, m, ch);
+            L.choff = (int64_t)tab.size();
+            for (auto& c : ch) tab.push_back(cpx<T>{(T)c.x, (T)c.y});
+            L.bsoff = (int64_t)tab.size();
+            for (auto& c : bs) tab.push_back(cpx<T>{(T)c.x, (T)c.y});
+            if (L.kind == RK_BLUE_S) {
+                L.lm = single_of(m);
+            } else {
+                RLen M{};
+                M.n = (int)m;
+                M.kind = RK_FOUR;
+                four_of(m, M);
+                lens.push_back(M);
+                L.lm = (int)lens.size() - 1;
+            }
+            lens.push_back(L);
+            id = (int)lens.size() - 1;
+        }
+        return idx_len[n] = id;
+    };
+
+    // per-transform descriptors, offsets, workspace
+    std::vector<int> tlen(count);
+    std::vector<int64_t> off(count), wsoff(count, 0);
+    int64_t tot = 0, wtot = 0;
+    for (int64_t i = 0; i < count; i++) {
+        if (lengths[i] < 1) throw JtError(ST_BAD_SIZE, "n must be greater than 0");
+        tlen[i] = len_of(lengths[i]);
+        off[i] = tot;
+        tot += lengths[i];
+        const RLen& L = lens[tlen[i]];
+        if (L.kind == RK_FOUR) {
+            wsoff[i] = wtot;
+            wtot += L.n;
+        } else if (L.kind == RK_BLUE_L) {
+            wsoff[i] = wtot;
+            wtot += 2 * (int64_t)L.m;
+        }
+    }
+    P->total = tot;
+    P->lens_of.assign(lengths, lengths + count);
+
+    // work items: cost-balanced phases interleaved with single items
+    std::vector<RItem> ph[4], singles;
+    const int CHUNK = (int)lim;
+    auto split_items = [&](std::vector<RItem>& v, int type, int len, int64_t tr, int total_lines, int line_len) {
+        int per = std::max(1, CHUNK / std::max(1, line_len));
+        for (int c = 0; c < total_lines; c += per) v.push_back(RItem{type, len, c, std::min(per, total_lines - c), tr});
+    };
+    std::vector<std::pair<double, RItem>> sgl;
+    for (int64_t i = 0; i < count; i++) {
+        const RLen& L = lens[tlen[i]];
+        if (L.kind == RK_SINGLE) {
+            sgl.push_back({(double)L.n * std::log2((double)L.n + 1), RItem{RT_SINGLE, tlen[i], 0, 1, i}});
+        } else if (L.kind == RK_BLUE_S) {
+            sgl.push_back({3.0 * L.m * std::log2((double)L.m), RItem{RT_BLUE_S, tlen[i], 0, 1, i}});
+        } else if (L.kind == RK_FOUR) {
+            split_items(ph[0], RT_FOUR_A, tlen[i], i, L.n2, L.n1);
+            split_items(ph[1], RT_FOUR_B, tlen[i], i, L.n1, L.n2);
+        } else {
+            const RLen& M = lens[L.lm];
+            split_items(ph[0], RT_BLUE_1, tlen[i], i, M.n2, M.n1);
+            split_items(ph[1], RT_BLUE_2, tlen[i], i, M.n1, M.n2);
+            split_items(ph[2], RT_BLUE_3, tlen[i], i, M.n2, M.n1);
+            split_items(ph[3], RT_BLUE_4, tlen[i], i, M.n1, M.n2);
+        }
+    }
+    std::stable_sort(sgl.begin(), sgl.end(), [](auto& a, auto& b) { return a.first > b.first; });
+    std::vector<RItem> items;
+    const size_t q = (sgl.size() + 3) / 4;
+    for (int p = 0; p < 4; p++) {
+        items.insert(items.end(), ph[p].begin(), ph[p].end());
+        for (size_t s = p * q; s < std::min(sgl.size(), (p + 1) * q); s++) items.push_back(sgl[s].second);
+    }
+    P->nitems = (int)items.size();
+
+    // upload
+    {
+        DevBuf& d = P->d_lens;
+        d.alloc(lens.size() * sizeof(RLen));
+        JT_HIP(hipMemcpy(d.p, lens.data(), d.bytes, hipMemcpyHostToDevice));
+    }
+    upload(P->d_tw, tab);
+    P->d_items.alloc(std::max<size_t>(1, items.size()) * sizeof(RItem));
+    if (!items.empty()) JT_HIP(hipMemcpy(P->d_items.p, items.data(), items.size() * sizeof(RItem), hipMemcpyHostToDevice));
+    P->d_off.alloc(count * sizeof(int64_t));
+    JT_HIP(hipMemcpy(P->d_off.p, off.data(), count * sizeof(int64_t), hipMemcpyHostToDevice));
+    P->d_wsoff.alloc(count * sizeof(int64_t));
+    JT_HIP(hipMemcpy(P->d_wsoff.p, wsoff.data(), count * sizeof(int64_t), hipMemcpyHostToDevice));
+    P->d_cnt.alloc(3 * count * sizeof(int));
+    P->d_next.alloc(sizeof(int));
+    P->d_err.alloc(sizeof(int));
+    JT_HIP(hipMemset(P->d_err.p, 0, sizeof(int)));
+    P->ws.alloc(std::max<int64_t>(wtot, 1) * sizeof(cpx<T>));
+
+    // launch shape: LDS for two buffers of the longest line; one persistent
+    // workgroup per resident slot
+    int mx = std::max(P->maxn, 16);
+    P->np = mx + (mx >> 5) + 1;
+    P->shm = 4 * (size_t)P->np * sizeof(T);
+    JT_HIP(hipFuncSetAttribute((const void*)ragged_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)P->shm));
+    int dev = 0, cus = 0, per = 0;
+    JT_HIP(hipGetDevice(&dev));
+    JT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    JT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)ragged_kernel<T>, 256, P->shm));
+    P->grid = std::max(1, cus * std::max(per, 1));
+    return P;
+}
+
+template <typename T>
+RArgs<T> ragged_args(RaggedPlan<T>& P, cpx<T>* data, int dir, bool scaled) {
+    RArgs<T> a;
+    a.data = data;
+    a.off = P.d_off.template as<int64_t>();
+    a.wsoff = P.d_wsoff.template as<int64_t>();
+    a.ws = P.ws.template as<cpx<T>>();
+    a.lens = P.d_lens.template as<RLen>();
+    a.tw = P.d_tw.template as<cpx<T>>();
+    a.items = P.d_items.template as<RItem>();
+    a.nitems = P.nitems;
+    a.next = P.d_next.template as<int>();
+    a.cnt = P.d_cnt.template as<int>();
+    a.err = P.d_err.template as<int>();
+    a.dir = dir;
+    a.scaled = scaled ? 1 : 0;
+    a.ntr = P.ntr;
+    return a;
+}
+
+template <typename T>
+void ragged_launch(RaggedPlan<T>& P, cpx<T>* data, int dir, bool scaled, hipStream_t st) {
+    RArgs<T> a = ragged_args(P, data, dir, scaled);
+    {
+        JT_PROF(st, "ragged_reset");
+        hipLaunchKernelGGL((ragged_reset<T>), ew_grid(3 * P.ntr + 1), dim3(256), 0, st, a);
+    }
+    JT_HIP(hipGetLastError());
+    {
+        JT_PROF(st, std::string("ragged<") + (sizeof(T) == 8 ? "f64" : "f32") + ",transforms=" +
+                        std::to_string(P.ntr) + ",points=" + std::to_string(P.total) + ">");
+        hipLaunchKernelGGL((ragged_kernel<T>), dim3(P.grid), dim3(256), P.shm, st, a, P.np);
+    }
+    JT_HIP(hipGetLastError());
+}
+
+// Runs through a cached hipGraph when the stream is not the null stream.
+template <typename T>
+void ragged_execute(RaggedPlan<T>& P, cpx<T>* data, int dir, bool scale, hipStream_t st, bool use_graph) {
+    const bool scaled = dir > 0 && scale;
+    if (!use_graph || st == nullptr) {
+        ragged_launch(P, data, dir, scaled, st);
+        return;
+    }
+    for (auto& g : P.graphs)
+        if (g.first.data == data && g.first.st == st && g.first.dir == dir && g.first.scale == scale) {
+            JT_HIP(hipGraphLaunch(g.second, st));
+            return;
+        }
+    hipGraph_t graph;
+    JT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        ragged_launch(P, data, dir, scaled, st);
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &graph);
+        throw;
+    }
+    JT_HIP(hipStreamEndCapture(st, &graph));
+    hipGraphExec_t exec;
+    JT_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    P.graphs.push_back({typename RaggedPlan<T>::GKey{data, st, dir, scale}, exec});
+    JT_HIP(hipGraphLaunch(exec, st));
+}
+
+}  // namespace jt
