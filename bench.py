@@ -134,6 +134,7 @@ def timed(torch, jt, calls, steps, warmup):
 
 def other_workload(args, torch, jt):
     """The other configurations: one JSON object each (not the headline line)."""
+    import numpy as np
     dev = torch.device("cuda:0")
     g = torch.Generator(device=dev)
     g.manual_seed(20110602)
@@ -198,6 +199,44 @@ def other_workload(args, torch, jt):
                     "step_GBps": 3 * 2 * n_pts * 8 / (ms * 1e-3) / 1e9})
         return out
 
+    if args.workload == "ragged":
+        from jtransforms_amd.workloads import ragged_lengths
+        lens = ragged_lengths(20000)
+        flops = float(np.sum(5.0 * lens * np.log2(lens)))
+        res = {}
+        for prec in ("f64", "f32"):
+            dt = torch.float64 if prec == "f64" else torch.float32
+            es = 16 if prec == "f64" else 8
+            # algorithmic bytes: read + write once per pass; 2 passes above 4096 points
+            passes = np.where(lens <= 4096, 1, 2)
+            byt = float(np.sum(2.0 * es * lens * passes))
+            for graph in (False, True):
+                print(f"ragged {prec} graph={graph}", file=sys.stderr, flush=True)
+                t0 = time.perf_counter()
+                plan = jt.RaggedBatch1D(lens, precision=prec, graph=graph)
+                t_plan = time.perf_counter() - t0
+                tot = int(2 * lens.sum())
+                base = torch.rand(tot, dtype=dt, device=dev, generator=g) * 2 - 1
+                nb = min(W + K, 6)
+                bufs = [base.clone() for _ in range(nb)]
+                st = torch.cuda.Stream()
+                with torch.cuda.stream(st):
+                    calls = [(lambda b=bufs[i % nb]: plan.complexForward(b)) for i in range(W + K)]
+                    ms, prof = timed(torch, jt, calls, K, W)
+                plan.sync_status()
+                key = f"{prec}_{'graph' if graph else 'direct'}"
+                rk = [v for k, v in prof.items() if k.startswith("ragged<")]
+                kern_ms = rk[0][1] / rk[0][0] if rk else None
+                res[key] = {"ms_per_step": ms, "GFLOPs": flops / (ms * 1e-3) / 1e9,
+                            "step_GBps_alg": byt / (ms * 1e-3) / 1e9, "plan_s": t_plan,
+                            "kernel_ms": kern_ms,
+                            "kernel_GBps_alg": byt / (kern_ms * 1e-3) / 1e9 if kern_ms else None}
+                del bufs, base, plan
+        out.update({"config": "ragged batch of 20000 1D complex transforms, n log-uniform in [16, 65536] "
+                              "(70% 2^a3^b5^c, 25% 2^k, 5% primes), one persistent launch",
+                    "total_points": int(lens.sum()), "flops": flops, "results": res})
+        return out
+
     # fft1d: n=1024 FP64, batch 1: complexForward + complexInverse(scale) + realForward
     n = 1024
     plan = jt.DoubleFFT_1D(n)
@@ -225,7 +264,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="fft2d",
-                    choices=["fft2d", "batch4096", "fft3d", "dct3d", "dst3d", "fft1d"],
+                    choices=["fft2d", "batch4096", "fft3d", "dct3d", "dst3d", "fft1d", "ragged"],
                     help="fft2d (default): the headline 4096x4096 FP64 line; the others "
                          "measure the remaining configurations")
     args = ap.parse_args()

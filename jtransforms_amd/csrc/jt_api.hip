@@ -53,6 +53,10 @@ template <typename F>
 int guarded(F&& fn) {
     try {
         g_err.clear();
+        // The HIP runtime may be shared with the caller (e.g. PyTorch); drop
+        // any error state its calls left behind so the checks after this
+        // library's own launches report only those launches.
+        (void)hipGetLastError();
         return fn();
     } catch (const JtError& e) {
         return set_err(e.code, e.what());

@@ -60,6 +60,7 @@ struct RArgs {
     const RLen* lens;
     const cpx<T>* tw;
     const RItem* items;
+    const int64_t* tlist;  // transform indices of the grouped single items
     int nitems;
     int* next;     // queue head
     int* cnt;      // 3 phase counters per transform
@@ -73,8 +74,81 @@ struct RArgs {
 // device helpers
 // ---------------------------------------------------------------------------
 
-// FFT of one line held in LDS (a buffers, gpos layout), runtime radices;
-// returns 0 if the result is in a, 1 if in b.
+// Transform `cnt` lines of length L.n held in LDS, G lines at a time: the
+// workgroup's threads split into G groups of tpl threads (tpl ~ the widest
+// stage's butterfly count), each group owning one line's slice of the four
+// LDS arrays (each array np_total reals).  load(line, k) / store(line, k, v)
+// move point k of line `line` in [0, cnt).  All threads take every barrier.
+template <typename T, bool LOAD_LF, bool STORE_LF, class LoadF, class StoreF>
+__device__ void rlines(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T* sm, int np_total, int cnt,
+                       LoadF load, StoreF store) {
+    const int n = L.n;
+    const int npl = n + (n >> 5) + 1;
+    int tpl = 1;
+    while (tpl < 256 && tpl * 16 < n) tpl <<= 1;
+    int G = 256 / tpl;
+    const int glds = np_total / npl;
+    if (G > glds) G = glds;
+    if (G < 1) G = 1;
+    const int t = threadIdx.x;
+    const int g = t / tpl, tl = t % tpl;
+    T* are = sm + g * npl;
+    T* aim = sm + np_total + g * npl;
+    T* bre = sm + 2 * np_total + g * npl;
+    T* bim = sm + 3 * np_total + g * npl;
+    for (int base = 0; base < cnt; base += G) {
+        const int line = base + g;
+        const bool act = g < G && line < cnt;
+        __syncthreads();
+        if constexpr (LOAD_LF) {
+            // lanes run across the G lines (adjacent columns in memory)
+            const int nl = min(G, cnt - base);
+            for (int idx = t; idx < nl * n; idx += 256) {
+                const int lg = idx % nl, k = idx / nl;
+                cpx<T> v = load(base + lg, k);
+                sm[lg * npl + gpos(k)] = v.x;
+                sm[np_total + lg * npl + gpos(k)] = v.y;
+            }
+        } else if (act) {
+            for (int k = tl; k < n; k += tpl) {
+                cpx<T> v = load(line, k);
+                are[gpos(k)] = v.x;
+                aim[gpos(k)] = v.y;
+            }
+        }
+        int ns = 1, which = 0;
+        for (int s = 0; s < L.nst; s++) {
+            __syncthreads();
+            const int R = L.radix[s];
+            if (act) {
+                const T* sr = which ? bre : are;
+                const T* si = which ? bim : aim;
+                T* dr = which ? are : bre;
+                T* di = which ? aim : bim;
+                if (dir < 0) gen_stage_rt<-1>(R, sr, si, dr, di, tw + L.twoff[s], n, ns, tl, tpl);
+                else gen_stage_rt<1>(R, sr, si, dr, di, tw + L.twoff[s], n, ns, tl, tpl);
+            }
+            ns *= R;
+            which ^= 1;
+        }
+        __syncthreads();
+        if constexpr (STORE_LF) {
+            const int nl = min(G, cnt - base);
+            const int o = which ? 2 * np_total : 0;
+            for (int idx = t; idx < nl * n; idx += 256) {
+                const int lg = idx % nl, k = idx / nl;
+                store(base + lg, k, cpx<T>{sm[o + lg * npl + gpos(k)], sm[o + np_total + lg * npl + gpos(k)]});
+            }
+        } else if (act) {
+            const T* rr = which ? bre : are;
+            const T* ri = which ? bim : aim;
+            for (int k = tl; k < n; k += tpl) store(line, k, cpx<T>{rr[gpos(k)], ri[gpos(k)]});
+        }
+    }
+}
+
+// FFT of one line held in LDS (a buffers, gpos layout) by the whole
+// workgroup; returns 0 if the result is in a, 1 if in b.
 template <typename T>
 __device__ int rline_fft(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T* are, T* aim, T* bre,
                          T* bim, int t, int nt) {
@@ -105,13 +179,16 @@ __device__ __forceinline__ cpx<T> rtw2(int dir, const cpx<T>* __restrict__ tw, c
 }
 
 // Wait until *c >= target (thread 0 polls, the workgroup then proceeds).
+// Polls are relaxed; one agent-scope acquire fence by thread 0 after the
+// condition holds makes the producers' data visible to the whole workgroup
+// (the caches it invalidates are shared by the workgroup's waves).
 // Bounded: after ~2 s of wall clock the wait gives up and flags an error.
 __device__ bool rwait(int* c, int target, int* err) {
     __shared__ int ok;
     if (threadIdx.x == 0) {
         int good = 1;
         uint64_t t0 = wall_clock64();
-        while (__hip_atomic_load(c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(8);
             if (wall_clock64() - t0 > 200000000ull) {  // 100 MHz counter: 2 s
                 good = 0;
@@ -119,17 +196,22 @@ __device__ bool rwait(int* c, int target, int* err) {
                 break;
             }
         }
+        __atomic_thread_fence(__ATOMIC_ACQUIRE);  // agent scope by default in HIP
         ok = good;
     }
     __syncthreads();
-    __threadfence();
     return ok != 0;
 }
 
+// Every thread waits for its own stores, then after the barrier one
+// agent-scope release fence and the counter increment by thread 0.
 __device__ __forceinline__ void rsignal(int* c, int add) {
-    __threadfence();
+    __builtin_amdgcn_s_waitcnt(0);  // this thread's stores have reached L2
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(c, add, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+        __atomic_thread_fence(__ATOMIC_RELEASE);
+        __hip_atomic_fetch_add(c, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 template <typename T>
@@ -167,16 +249,15 @@ __global__ void __launch_bounds__(256) ragged_kernel(RArgs<T> a, int np) {
         int* c3 = a.cnt + 3 * tr;
         switch (item.type) {
             case RT_SINGLE: {
-                const int n = L.n;
-                for (int k = t; k < n; k += nt) {
-                    cpx<T> v = x[k];
-                    are[gpos(k)] = v.x;
-                    aim[gpos(k)] = v.y;
-                }
-                int w = rline_fft<T>(dir, L, a.tw, are, aim, bre, bim, t, nt);
-                const T* rr = w ? bre : are;
-                const T* ri = w ? bim : aim;
-                for (int k = t; k < n; k += nt) x[k] = cpx<T>{rr[gpos(k)] * scale, ri[gpos(k)] * scale};
+                // cnt transforms of this length, listed from tlist[c0]
+                const int64_t* tl0 = a.tlist + item.c0;
+                const cpx<T>* data = a.data;
+                const int64_t* off = a.off;
+                rlines<T, false, false>(dir, L, a.tw, sm, np, item.cnt,
+                          [&](int line, int k) { return data[off[tl0[line]] + k]; },
+                          [&](int line, int k, cpx<T> v) {
+                              a.data[off[tl0[line]] + k] = cpx<T>{v.x * scale, v.y * scale};
+                          });
                 break;
             }
             case RT_FOUR_A:
@@ -187,38 +268,27 @@ __global__ void __launch_bounds__(256) ragged_kernel(RArgs<T> a, int np) {
                 const bool blue = item.type != RT_FOUR_A;
                 const RLen& B = blue ? a.lens[L.lm] : L;  // the four-step length
                 const RLen& L1 = a.lens[B.l1];
-                const int n1 = B.n1, n2 = B.n2;
-                const int64_t N = (int64_t)n1 * n2;
+                const int n2 = B.n2;
+                const int64_t N = (int64_t)B.n1 * n2;
                 const int d = item.type == RT_BLUE_1 ? -1 : (item.type == RT_BLUE_3 ? 1 : dir);
                 cpx<T>* wsA = a.ws + a.wsoff[tr];
-                const cpx<T>* src = item.type == RT_FOUR_A ? x : (item.type == RT_BLUE_1 ? x : wsA + N);
+                const cpx<T>* src = item.type == RT_BLUE_3 ? wsA + N : x;
                 if (item.type == RT_BLUE_3 && !rwait(c3 + 1, B.n1, a.err)) break;
-                for (int j2 = item.c0; j2 < item.c0 + item.cnt; j2++) {
-                    __syncthreads();  // the previous line's results have been read
-                    for (int j1 = t; j1 < n1; j1 += nt) {
-                        int64_t e = (int64_t)j1 * n2 + j2;
-                        cpx<T> v;
-                        if (item.type == RT_BLUE_1) {
-                            v = cpx<T>{T(0), T(0)};
-                            if (e < L.n) {
-                                cpx<T> c = a.tw[L.choff + e];
-                                v = dir < 0 ? cmulc(src[e], c) : cmul(src[e], c);
-                            }
-                        } else {
-                            v = src[e];
-                        }
-                        are[gpos(j1)] = v.x;
-                        aim[gpos(j1)] = v.y;
-                    }
-                    int w = rline_fft<T>(d, L1, a.tw, are, aim, bre, bim, t, nt);
-                    const T* rr = w ? bre : are;
-                    const T* ri = w ? bim : aim;
-                    for (int k1 = t; k1 < n1; k1 += nt) {
-                        cpx<T> v = {rr[gpos(k1)], ri[gpos(k1)]};
-                        v = cmul(v, rtw2<T>(d, a.tw, B, (int64_t)j2 * k1, N));
-                        wsA[(int64_t)k1 * n2 + j2] = v;
-                    }
-                }
+                const int type = item.type, c0 = item.c0, nreal = L.n;
+                const cpx<T>* chirp = a.tw + L.choff;
+                rlines<T, true, true>(d, L1, a.tw, sm, np, item.cnt,
+                          [&](int line, int j1) {
+                              int64_t e = (int64_t)j1 * n2 + (c0 + line);
+                              if (type == RT_BLUE_1) {
+                                  if (e >= nreal) return cpx<T>{T(0), T(0)};
+                                  return dir < 0 ? cmulc(src[e], chirp[e]) : cmul(src[e], chirp[e]);
+                              }
+                              return src[e];
+                          },
+                          [&](int line, int k1, cpx<T> v) {
+                              int j2 = c0 + line;
+                              wsA[(int64_t)k1 * n2 + j2] = cmul(v, rtw2<T>(d, a.tw, B, (int64_t)j2 * k1, N));
+                          });
                 rsignal(item.type == RT_BLUE_3 ? c3 + 2 : c3 + 0, item.cnt);
                 break;
             }
@@ -234,33 +304,26 @@ __global__ void __launch_bounds__(256) ragged_kernel(RArgs<T> a, int np) {
                 const int64_t N = (int64_t)n1 * n2;
                 const int d = item.type == RT_BLUE_2 ? -1 : (item.type == RT_BLUE_4 ? 1 : dir);
                 cpx<T>* wsA = a.ws + a.wsoff[tr];
-                const int* wc = item.type == RT_BLUE_4 ? c3 + 2 : c3 + 0;
-                if (!rwait(const_cast<int*>(wc), n2, a.err)) break;
-                for (int k1 = item.c0; k1 < item.c0 + item.cnt; k1++) {
-                    __syncthreads();  // the previous line's results have been read
-                    for (int j2 = t; j2 < n2; j2 += nt) {
-                        cpx<T> v = wsA[(int64_t)k1 * n2 + j2];
-                        are[gpos(j2)] = v.x;
-                        aim[gpos(j2)] = v.y;
-                    }
-                    int w = rline_fft<T>(d, L2, a.tw, are, aim, bre, bim, t, nt);
-                    const T* rr = w ? bre : are;
-                    const T* ri = w ? bim : aim;
-                    for (int k2 = t; k2 < n2; k2 += nt) {
-                        int64_t K = k1 + (int64_t)n1 * k2;
-                        cpx<T> v = {rr[gpos(k2)], ri[gpos(k2)]};
-                        if (item.type == RT_FOUR_B) {
-                            x[K] = cpx<T>{v.x * scale, v.y * scale};
-                        } else if (item.type == RT_BLUE_2) {
-                            cpx<T> b = a.tw[L.bsoff + K];
-                            wsA[N + K] = dir < 0 ? cmul(v, b) : cmulc(v, b);
-                        } else if (K < L.n) {
-                            cpx<T> c = a.tw[L.choff + K];
-                            v = dir < 0 ? cmulc(v, c) : cmul(v, c);
-                            x[K] = cpx<T>{v.x * scale, v.y * scale};
-                        }
-                    }
-                }
+                int* wc = item.type == RT_BLUE_4 ? c3 + 2 : c3 + 0;
+                if (!rwait(wc, n2, a.err)) break;
+                const int type = item.type, c0 = item.c0, nreal = L.n;
+                const cpx<T>* chirp = a.tw + L.choff;
+                const cpx<T>* bspec = a.tw + L.bsoff;
+                rlines<T, false, true>(d, L2, a.tw, sm, np, item.cnt,
+                          [&](int line, int j2) { return wsA[(int64_t)(c0 + line) * n2 + j2]; },
+                          [&](int line, int k2, cpx<T> v) {
+                              int64_t K = (c0 + line) + (int64_t)n1 * k2;
+                              if (type == RT_FOUR_B) {
+                                  x[K] = cpx<T>{v.x * scale, v.y * scale};
+                              } else if (type == RT_BLUE_2) {
+                                  cpx<T> bb = bspec[K];
+                                  wsA[N + K] = dir < 0 ? cmul(v, bb) : cmulc(v, bb);
+                              } else if (K < nreal) {
+                                  cpx<T> c = chirp[K];
+                                  v = dir < 0 ? cmulc(v, c) : cmul(v, c);
+                                  x[K] = cpx<T>{v.x * scale, v.y * scale};
+                              }
+                          });
                 if (item.type == RT_BLUE_2) rsignal(c3 + 1, item.cnt);
                 break;
             }
@@ -315,7 +378,7 @@ struct RaggedPlan {
     int maxn = 0;                // largest LDS line length
     int nitems = 0;
     std::vector<int64_t> lens_of;  // per transform
-    DevBuf d_lens, d_tw, d_items, d_off, d_wsoff, d_cnt, d_next, d_err, ws;
+    DevBuf d_lens, d_tw, d_items, d_tlist, d_off, d_wsoff, d_cnt, d_next, d_err, ws;
     int grid = 0;
     size_t shm = 0;
     int np = 0;
@@ -492,16 +555,19 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
 
     // work items: cost-balanced phases interleaved with single items
     std::vector<RItem> ph[4], singles;
-    const int CHUNK = (int)lim;
+    // four-step items carry ~4 LDS loads of lines each
+    const int CHUNK = (int)(4 * lim);
     auto split_items = [&](std::vector<RItem>& v, int type, int len, int64_t tr, int total_lines, int line_len) {
         int per = std::max(1, CHUNK / std::max(1, line_len));
         for (int c = 0; c < total_lines; c += per) v.push_back(RItem{type, len, c, std::min(per, total_lines - c), tr});
     };
     std::vector<std::pair<double, RItem>> sgl;
+    std::map<int, std::vector<int64_t>> by_len;  // single-kernel transforms grouped by length
+    std::vector<int64_t> tlist;
     for (int64_t i = 0; i < count; i++) {
         const RLen& L = lens[tlen[i]];
         if (L.kind == RK_SINGLE) {
-            sgl.push_back({(double)L.n * std::log2((double)L.n + 1), RItem{RT_SINGLE, tlen[i], 0, 1, i}});
+            by_len[tlen[i]].push_back(i);
         } else if (L.kind == RK_BLUE_S) {
             sgl.push_back({3.0 * L.m * std::log2((double)L.m), RItem{RT_BLUE_S, tlen[i], 0, 1, i}});
         } else if (L.kind == RK_FOUR) {
@@ -513,6 +579,17 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
             split_items(ph[1], RT_BLUE_2, tlen[i], i, M.n1, M.n2);
             split_items(ph[2], RT_BLUE_3, tlen[i], i, M.n2, M.n1);
             split_items(ph[3], RT_BLUE_4, tlen[i], i, M.n1, M.n2);
+        }
+    }
+    for (auto& kv : by_len) {
+        const RLen& L = lens[kv.first];
+        const int per = std::max(1, (int)(CHUNK / std::max(1, L.n)));
+        const auto& v = kv.second;
+        for (size_t c = 0; c < v.size(); c += per) {
+            int cnt = (int)std::min<size_t>(per, v.size() - c);
+            RItem it{RT_SINGLE, kv.first, (int)tlist.size(), cnt, v[c]};
+            for (int q = 0; q < cnt; q++) tlist.push_back(v[c + q]);
+            sgl.push_back({(double)cnt * L.n * std::log2((double)L.n + 1), it});
         }
     }
     std::stable_sort(sgl.begin(), sgl.end(), [](auto& a, auto& b) { return a.first > b.first; });
@@ -533,6 +610,9 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
     upload(P->d_tw, tab);
     P->d_items.alloc(std::max<size_t>(1, items.size()) * sizeof(RItem));
     if (!items.empty()) JT_HIP(hipMemcpy(P->d_items.p, items.data(), items.size() * sizeof(RItem), hipMemcpyHostToDevice));
+    P->d_tlist.alloc(std::max<size_t>(1, tlist.size()) * sizeof(int64_t));
+    if (!tlist.empty())
+        JT_HIP(hipMemcpy(P->d_tlist.p, tlist.data(), tlist.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     P->d_off.alloc(count * sizeof(int64_t));
     JT_HIP(hipMemcpy(P->d_off.p, off.data(), count * sizeof(int64_t), hipMemcpyHostToDevice));
     P->d_wsoff.alloc(count * sizeof(int64_t));
@@ -568,6 +648,7 @@ RArgs<T> ragged_args(RaggedPlan<T>& P, cpx<T>* data, int dir, bool scaled) {
     a.lens = P.d_lens.template as<RLen>();
     a.tw = P.d_tw.template as<cpx<T>>();
     a.items = P.d_items.template as<RItem>();
+    a.tlist = P.d_tlist.template as<int64_t>();
     a.nitems = P.nitems;
     a.next = P.d_next.template as<int>();
     a.cnt = P.d_cnt.template as<int>();
