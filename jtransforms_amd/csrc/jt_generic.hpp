@@ -19,6 +19,17 @@ struct GenPlan {
 
 __device__ __forceinline__ int gpos(int k) { return k + (k >> 5); }
 
+// q = a / d for 0 <= a < 2^22, d >= 1, from a float reciprocal with one
+// correction step (exact: the float quotient a*(1/d) is within 3*2^-24 a/d
+// of the true one, i.e. within 1 for these ranges)
+__device__ __forceinline__ int fdiv(int a, int d, float rcp) {
+    int q = (int)((float)a * rcp);
+    int r = a - q * d;
+    if (r < 0) q--;
+    else if (r >= d) q++;
+    return q;
+}
+
 // Prime radix P (11, 13): direct O(P^2) evaluation; exp(DIR 2 pi i m/P) for
 // m in [0,P) is evaluated once per butterfly.
 template <int P, int DIR, typename T>
@@ -55,8 +66,10 @@ __device__ __forceinline__ void gen_stage(const T* are, const T* aim, T* bre, T*
                                           const cpx<T>* __restrict__ tws, int n, int ns, int t,
                                           int nt) {
     const int m = n / R;
+    const float rns = 1.0f / (float)ns;
     for (int j = t; j < m; j += nt) {
-        const int jm = j % ns;
+        const int jh = fdiv(j, ns, rns);
+        const int jm = j - jh * ns;
         cpx<T> v[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -68,7 +81,7 @@ __device__ __forceinline__ void gen_stage(const T* are, const T* aim, T* bre, T*
             for (int r = 1; r < R; r++) v[r] = cmul(v[r], tw_dir<DIR>(tws[jm * (R - 1) + r - 1]));
         }
         dft_any<R, DIR>(v);
-        const int d = (j / ns) * ns * R + jm;
+        const int d = jh * ns * R + jm;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             int k = gpos(d + r * ns);

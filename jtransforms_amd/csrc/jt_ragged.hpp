@@ -101,19 +101,47 @@ __device__ void rlines(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T*
         const bool act = g < G && line < cnt;
         __syncthreads();
         if constexpr (LOAD_LF) {
-            // lanes run across the G lines (adjacent columns in memory)
+            // lanes run across the G lines (adjacent columns in memory);
+            // four loads in flight per thread before the LDS writes
             const int nl = min(G, cnt - base);
-            for (int idx = t; idx < nl * n; idx += 256) {
-                const int lg = idx % nl, k = idx / nl;
-                cpx<T> v = load(base + lg, k);
-                sm[lg * npl + gpos(k)] = v.x;
-                sm[np_total + lg * npl + gpos(k)] = v.y;
+            const int tot = nl * n;
+            const float rnl = 1.0f / (float)nl;
+            for (int i0 = t; i0 < tot; i0 += 4 * 256) {
+                cpx<T> v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int idx = i0 + u * 256;
+                    if (idx < tot) {
+                        const int k = fdiv(idx, nl, rnl);
+                        v[u] = load(base + idx - k * nl, k);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int idx = i0 + u * 256;
+                    if (idx < tot) {
+                        const int k = fdiv(idx, nl, rnl), lg = idx - k * nl;
+                        sm[lg * npl + gpos(k)] = v[u].x;
+                        sm[np_total + lg * npl + gpos(k)] = v[u].y;
+                    }
+                }
             }
         } else if (act) {
-            for (int k = tl; k < n; k += tpl) {
-                cpx<T> v = load(line, k);
-                are[gpos(k)] = v.x;
-                aim[gpos(k)] = v.y;
+            for (int k0 = tl; k0 < n; k0 += 4 * tpl) {
+                cpx<T> v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int k = k0 + u * tpl;
+                    if (k < n) v[u] = load(line, k);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int k = k0 + u * tpl;
+                    if (k < n) {
+                        are[gpos(k)] = v[u].x;
+                        aim[gpos(k)] = v[u].y;
+                    }
+                }
             }
         }
         int ns = 1, which = 0;
@@ -135,8 +163,9 @@ __device__ void rlines(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T*
         if constexpr (STORE_LF) {
             const int nl = min(G, cnt - base);
             const int o = which ? 2 * np_total : 0;
+            const float rnl = 1.0f / (float)nl;
             for (int idx = t; idx < nl * n; idx += 256) {
-                const int lg = idx % nl, k = idx / nl;
+                const int k = fdiv(idx, nl, rnl), lg = idx - k * nl;
                 store(base + lg, k, cpx<T>{sm[o + lg * npl + gpos(k)], sm[o + np_total + lg * npl + gpos(k)]});
             }
         } else if (act) {
@@ -173,7 +202,7 @@ __device__ int rline_fft(int dir, const RLen& L, const cpx<T>* __restrict__ tw, 
 template <typename T>
 __device__ __forceinline__ cpx<T> rtw2(int dir, const cpx<T>* __restrict__ tw, const RLen& L, int64_t e,
                                        int64_t N) {
-    e %= N;
+    (void)N;  // e = j2*k1 < n2*n1 = N: no reduction needed
     cpx<T> w = cmul(tw[L.hioff + (e >> 8)], tw[L.looff + (e & 255)]);
     return dir < 0 ? w : cpx<T>{w.x, -w.y};
 }
@@ -278,7 +307,7 @@ __global__ void __launch_bounds__(256) ragged_kernel(RArgs<T> a, int np) {
                 const cpx<T>* chirp = a.tw + L.choff;
                 rlines<T, true, true>(d, L1, a.tw, sm, np, item.cnt,
                           [&](int line, int j1) {
-                              int64_t e = (int64_t)j1 * n2 + (c0 + line);
+                              const int e = j1 * n2 + (c0 + line);
                               if (type == RT_BLUE_1) {
                                   if (e >= nreal) return cpx<T>{T(0), T(0)};
                                   return dir < 0 ? cmulc(src[e], chirp[e]) : cmul(src[e], chirp[e]);
@@ -287,7 +316,7 @@ __global__ void __launch_bounds__(256) ragged_kernel(RArgs<T> a, int np) {
                           },
                           [&](int line, int k1, cpx<T> v) {
                               int j2 = c0 + line;
-                              wsA[(int64_t)k1 * n2 + j2] = cmul(v, rtw2<T>(d, a.tw, B, (int64_t)j2 * k1, N));
+                              wsA[k1 * n2 + j2] = cmul(v, rtw2<T>(d, a.tw, B, j2 * k1, N));
                           });
                 rsignal(item.type == RT_BLUE_3 ? c3 + 2 : c3 + 0, item.cnt);
                 break;
@@ -310,9 +339,9 @@ __global__ void __launch_bounds__(256) ragged_kernel(RArgs<T> a, int np) {
                 const cpx<T>* chirp = a.tw + L.choff;
                 const cpx<T>* bspec = a.tw + L.bsoff;
                 rlines<T, false, true>(d, L2, a.tw, sm, np, item.cnt,
-                          [&](int line, int j2) { return wsA[(int64_t)(c0 + line) * n2 + j2]; },
+                          [&](int line, int j2) { return wsA[(c0 + line) * n2 + j2]; },
                           [&](int line, int k2, cpx<T> v) {
-                              int64_t K = (c0 + line) + (int64_t)n1 * k2;
+                              const int K = (c0 + line) + n1 * k2;
                               if (type == RT_FOUR_B) {
                                   x[K] = cpx<T>{v.x * scale, v.y * scale};
                               } else if (type == RT_BLUE_2) {
