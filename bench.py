@@ -213,7 +213,9 @@ def other_workload(args, torch, jt):
             for graph in (False, True):
                 print(f"ragged {prec} graph={graph}", file=sys.stderr, flush=True)
                 t0 = time.perf_counter()
-                plan = jt.RaggedBatch1D(lens, precision=prec, graph=graph)
+                # the first creation per precision runs the autotuner; the
+                # second reuses the schedule it chose
+                plan = jt.RaggedBatch1D(lens, precision=prec, graph=graph, autotune=True)
                 t_plan = time.perf_counter() - t0
                 tot = int(2 * lens.sum())
                 base = torch.rand(tot, dtype=dt, device=dev, generator=g) * 2 - 1
@@ -229,6 +231,7 @@ def other_workload(args, torch, jt):
                 kern_ms = rk[0][1] / rk[0][0] if rk else None
                 res[key] = {"ms_per_step": ms, "GFLOPs": flops / (ms * 1e-3) / 1e9,
                             "step_GBps_alg": byt / (ms * 1e-3) / 1e9, "plan_s": t_plan,
+                            "schedule": plan.describe(),
                             "kernel_ms": kern_ms,
                             "kernel_GBps_alg": byt / (kern_ms * 1e-3) / 1e9 if kern_ms else None}
                 del bufs, base, plan

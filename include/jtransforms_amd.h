@@ -56,7 +56,11 @@ enum {
     JT_FLAG_SPLIT_FP16 = 1,
     /* ragged batches: capture the launch sequence in a hipGraph on the first
      * call for a given (data, stream, op) and replay it afterwards */
-    JT_FLAG_GRAPH = 2
+    JT_FLAG_GRAPH = 2,
+    /* ragged batches: choose the schedule (LDS split, tile shape, work-item
+     * size, radix order) by timing candidate plans at creation; the choice is
+     * remembered for the same length list within the process */
+    JT_FLAG_AUTOTUNE = 4
 };
 
 /* operations (one per reference method family) */
@@ -113,6 +117,12 @@ int jt_plan_create_fft1d_batch(jt_plan** out, int precision, int64_t n, int64_t 
  * transform. */
 int jt_plan_create_ragged(jt_plan** out, int precision, int64_t count, const int64_t* lengths,
                           uint32_t flags);
+
+/* Writes a one-line description of the plan (its kernels' plan kinds; for a
+ * ragged batch its schedule and, when autotuned, the timed candidates) into
+ * buf (NUL-terminated, truncated to len bytes) and returns the full length.
+ * No counterpart in the reference. */
+int64_t jt_plan_describe(const jt_plan* plan, char* buf, int64_t len);
 
 /* Waits for the device and reports a failure recorded on it by the plan's
  * last launches (a ragged batch whose dependency wait timed out). */

@@ -33,6 +33,7 @@ JT_OK, JT_BAD_SIZE, JT_NOT_POW2, JT_TOO_BIG, JT_HIP_ERROR, JT_OOM, JT_BAD_ARG, J
 JT_F64, JT_F32 = 0, 1
 JT_FLAG_SPLIT_FP16 = 1
 JT_FLAG_GRAPH = 2
+JT_FLAG_AUTOTUNE = 4
 (COMPLEX_FORWARD, COMPLEX_INVERSE, REAL_FORWARD, REAL_INVERSE, REAL_FORWARD_FULL,
  REAL_INVERSE_FULL, FORWARD, INVERSE) = range(8)
 
@@ -41,7 +42,7 @@ EXPORTED_SYMBOLS = [
     "jt_plan_create_dst", "jt_plan_create_dht", "jt_plan_create_fft1d_batch", "jt_plan_destroy",
     "jt_execute", "jt_execute_host", "jt_plan_elements", "jt_status_string", "jt_last_error",
     "jt_version", "jt_profile_begin", "jt_profile_end", "jt_plan_set_flags",
-    "jt_plan_create_ragged", "jt_plan_sync_status",
+    "jt_plan_create_ragged", "jt_plan_sync_status", "jt_plan_describe",
 ]
 
 
@@ -84,6 +85,8 @@ def lib():
         L.jt_plan_set_flags.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
         L.jt_plan_create_ragged.argtypes = [P, ctypes.c_int, i64, ctypes.POINTER(i64), ctypes.c_uint32]
         L.jt_plan_sync_status.argtypes = [ctypes.c_void_p]
+        L.jt_plan_describe.argtypes = [ctypes.c_void_p, ctypes.c_char_p, i64]
+        L.jt_plan_describe.restype = i64
         L.jt_execute.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, i64, ctypes.c_int,
                                  ctypes.c_void_p]
         L.jt_execute_host.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, i64,
@@ -141,6 +144,13 @@ class _Plan:
     @property
     def dtype(self):
         return np.float64 if self._precision == JT_F64 else np.float32
+
+    def describe(self) -> str:
+        """The library's one-line description of this plan (schedule, tuning)."""
+        n = lib().jt_plan_describe(self._h, None, 0)
+        buf = ctypes.create_string_buffer(int(n) + 1)
+        lib().jt_plan_describe(self._h, buf, n + 1)
+        return buf.value.decode()
 
     def elements(self, op: int) -> int:
         return int(lib().jt_plan_elements(self._h, op))
@@ -269,13 +279,13 @@ class RaggedBatch1D(_Plan):
     in one persistent launch (no reference counterpart: it replaces one
     DoubleFFT_1D per length).  graph=True replays a captured hipGraph."""
 
-    def __init__(self, lengths, precision: str = "f64", graph: bool = False):
+    def __init__(self, lengths, precision: str = "f64", graph: bool = False, autotune: bool = False):
         self._precision = JT_F64 if precision in ("f64", "double") else JT_F32
         self.lengths = np.ascontiguousarray(lengths, dtype=np.int64)
         self.offsets = np.concatenate([[0], np.cumsum(2 * self.lengths)])
         arr = self.lengths.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
-        super().__init__(lib().jt_plan_create_ragged, len(self.lengths), arr,
-                         JT_FLAG_GRAPH if graph else 0)
+        flags = (JT_FLAG_GRAPH if graph else 0) | (JT_FLAG_AUTOTUNE if autotune else 0)
+        super().__init__(lib().jt_plan_create_ragged, len(self.lengths), arr, flags)
 
     def complexForward(self, a):
         self._run(COMPLEX_FORWARD, a, 0)

@@ -2,7 +2,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/jtransforms_amd.h"
 #include "jt_multid.hpp"
@@ -31,6 +34,7 @@ struct Impl {
 }  // namespace
 
 struct jt_plan {
+    std::string desc;  // tuning record (ragged batches)
     int family = FAM_FFT;
     int precision = JT_F64;
     int ndim = 1;
@@ -107,8 +111,13 @@ int64_t ws_need(jt_plan* p) {
 template <typename T>
 void apply_flags(Impl<T>& I, uint32_t flags) {
     const bool mf = sizeof(T) == 4 && (flags & JT_FLAG_SPLIT_FP16);
-    for (int a = 0; a < 3; a++)
-        if (I.ax[a]) I.ax[a]->mf = mf && I.ax[a]->kind == K_TPL;
+    for (int a = 0; a < 3; a++) {
+        if (!I.ax[a]) continue;
+        Fft1D<T>& p = *I.ax[a];
+        std::vector<int> r;
+        p.mf = mf && p.kind == K_TPL && mf_radices(p.n, r);
+        if (p.mf && !p.tw_mf.p) upload(p.tw_mf, stage_twiddles_mf<T>(r));
+    }
 }
 
 template <typename T>
@@ -384,13 +393,61 @@ int jt_plan_create_fft1d_batch(jt_plan** out, int precision, int64_t n, int64_t 
     return create(out, FAM_FFT, precision, 1, &n, batch, flags);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Tuned schedules by (precision, length list), for the life of the process.
+std::mutex g_tune_mu;
+std::map<std::pair<int, std::vector<int64_t>>, RTune> g_tuned;
+
+template <typename T>
+std::unique_ptr<RaggedPlan<T>> make_ragged_tuned(const int64_t* lengths, int64_t count, bool tune,
+                                                 std::string* desc) {
+    std::unique_ptr<RaggedPlan<T>> P;
+    if (!tune) {
+        P = make_ragged<T>(lengths, count);
+    } else {
+        auto key = std::make_pair((int)sizeof(T), std::vector<int64_t>(lengths, lengths + count));
+        RTune known;
+        bool have = false;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tuned.find(key);
+            if (it != g_tuned.end()) {
+                known = it->second;
+                have = true;
+            }
+        }
+        if (have) {
+            P = make_ragged<T>(lengths, count, known);
+            *desc = "autotuned (remembered)";
+        } else {
+            std::string log;
+            P = ragged_autotune<T>(lengths, count, &log);
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tuned[key] = P->tune;
+            for (auto& c : log)
+                if (c == '\n') c = ';';
+            *desc = "autotuned: " + log;
+        }
+    }
+    *desc = std::string("ragged ") + (sizeof(T) == 8 ? "f64" : "f32") + " transforms=" + std::to_string(P->ntr) +
+            " points=" + std::to_string(P->total) + " schedule " + P->tune.str() + (desc->empty() ? "" : " ") + *desc;
+    return P;
+}
+
+}  // namespace
+
+extern "C" {
+
 int jt_plan_create_ragged(jt_plan** out, int precision, int64_t count, const int64_t* lengths, uint32_t flags) {
     return guarded([&]() -> int {
         if (!out) return set_err(JT_BAD_ARG, "null plan pointer");
         *out = nullptr;
         if (precision != JT_F64 && precision != JT_F32) return set_err(JT_BAD_ARG, "unknown precision");
         if (count < 1 || !lengths) return set_err(JT_BAD_SIZE, "count must be greater than 0");
-        if (flags & ~(uint32_t)JT_FLAG_GRAPH) return set_err(JT_BAD_ARG, "unknown flags");
+        if (flags & ~(uint32_t)(JT_FLAG_GRAPH | JT_FLAG_AUTOTUNE)) return set_err(JT_BAD_ARG, "unknown flags");
         for (int64_t i = 0; i < count; i++)
             if (lengths[i] < 1) return set_err(JT_BAD_SIZE, "n must be greater than 0");
         auto p = std::make_unique<jt_plan>();
@@ -399,16 +456,35 @@ int jt_plan_create_ragged(jt_plan** out, int precision, int64_t count, const int
         p->ndim = 1;
         p->dims[0] = count;
         p->flags = flags;
+        const bool tune = (flags & JT_FLAG_AUTOTUNE) != 0;
         if (precision == JT_F64) {
             p->d = std::make_unique<Impl<double>>();
-            p->d->ragged = make_ragged<double>(lengths, count);
+            p->d->ragged = make_ragged_tuned<double>(lengths, count, tune, &p->desc);
         } else {
             p->f = std::make_unique<Impl<float>>();
-            p->f->ragged = make_ragged<float>(lengths, count);
+            p->f->ragged = make_ragged_tuned<float>(lengths, count, tune, &p->desc);
         }
         *out = p.release();
         return JT_OK;
     });
+}
+
+int64_t jt_plan_describe(const jt_plan* p, char* buf, int64_t len) {
+    if (!p) return -1;
+    std::string d = p->desc;
+    if (d.empty()) {
+        d = std::string(p->family == FAM_FFT ? "fft" : p->family == FAM_DCT ? "dct" : p->family == FAM_DST ? "dst" : "dht") +
+            (p->precision == JT_F64 ? " f64" : " f32") + " dims=";
+        for (int i = 0; i < p->ndim; i++) d += (i ? "x" : "") + std::to_string(p->dims[i]);
+        if (p->batch > 1) d += " batch=" + std::to_string(p->batch);
+        if (p->flags & JT_FLAG_SPLIT_FP16) d += " split-fp16";
+    }
+    if (buf && len > 0) {
+        size_t n = std::min<size_t>((size_t)len - 1, d.size());
+        std::memcpy(buf, d.data(), n);
+        buf[n] = 0;
+    }
+    return (int64_t)d.size();
 }
 
 int jt_plan_sync_status(jt_plan* p) {

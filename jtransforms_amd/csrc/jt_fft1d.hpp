@@ -22,6 +22,7 @@
 #include "jt_prof.hpp"
 #include "jt_io.hpp"
 #include "jt_stockham.hpp"
+#include "jt_mfstockham.hpp"
 
 namespace jt {
 
@@ -126,6 +127,23 @@ inline bool tpl_radices(int64_t n, std::vector<int>& r) {
     }
 }
 
+// radix lists of the split-FP16 MFMA kernels: radix-16 stages first, one
+// closing radix 2/4/8 stage when N is not a power of 16
+inline bool mf_radices(int64_t n, std::vector<int>& r) {
+    switch (n) {
+        case 4096: r = {16, 16, 16}; return true;
+        case 2048: r = {16, 16, 8}; return true;
+        case 1024: r = {16, 16, 4}; return true;
+        case 512: r = {16, 16, 2}; return true;
+        case 256: r = {16, 16}; return true;
+        case 128: r = {16, 8}; return true;
+        case 64: r = {16, 4}; return true;
+        case 32: r = {16, 2}; return true;
+        case 16: r = {16}; return true;
+        default: return false;
+    }
+}
+
 template <typename T>
 inline int64_t gen_max() { return sizeof(T) == 8 ? 4096 : 8192; }
 
@@ -140,6 +158,21 @@ std::vector<cpx<T>> stage_twiddles(const std::vector<int>& radices, std::vector<
         ns *= R;
     }
     if (tw.empty()) tw.push_back(cpx<T>{T(1), T(0)});
+    return tw;
+}
+
+// Twiddles of the split-FP16 MFMA kernels (layout of mf_twoff): 16 factors
+// per jm (r = 0..15) for every radix-16 stage, R-1 for a closing radix R.
+template <typename T>
+std::vector<cpx<T>> stage_twiddles_mf(const std::vector<int>& radices) {
+    std::vector<cpx<T>> tw;
+    int64_t ns = 1;
+    for (int R : radices) {
+        const int r0 = R == 16 ? 0 : 1;
+        for (int64_t jm = 0; jm < ns; jm++)
+            for (int r = r0; r < R; r++) tw.push_back(expm<T>(jm * r, ns * R));
+        ns *= R;
+    }
     return tw;
 }
 
@@ -214,6 +247,7 @@ struct Fft1D {
     int64_t n = 0;
     int kind = K_GEN;
     bool mf = false;  // split-FP16 MFMA radix-16 stages (FP32 compile-time plans)
+    DevBuf tw_mf;     // stage twiddles of mf_radices(n), built when mf is first set
     std::vector<int> radices;
     GenPlan gp{};
     DevBuf tw;
@@ -359,10 +393,47 @@ bool dispatch_tpl_mf(int64_t n, IO io, const cpx<T>* tw, hipStream_t st) {
     }
 }
 
+// split-FP16 MFMA kernels: W lines per workgroup, N/16 threads per line
+template <int N, bool COL>
+struct MfShape {
+    static constexpr int TL = N / 16;
+    static constexpr int W = COL ? TplShape<float, N, TL, true>::W : (256 / TL > 0 ? 256 / TL : 1);
+};
+
+template <int DIR, int N, bool COL, class IO, int... Rs>
+void go_mf(IO io, const cpx<float>* tw, hipStream_t st) {
+    constexpr int W = MfShape<N, COL>::W;
+    constexpr int TL = MfShape<N, COL>::TL;
+    int64_t groups = (io.nlines + W - 1) / W;
+    if (groups <= 0) return;
+    JT_PROF(st, std::string("stockham<f32,N=") + std::to_string(N) + (COL ? ",col" : ",row") + ",mfma,W=" +
+                    std::to_string(W) + ",lines=" + std::to_string(io.nlines) + "," + IO::tag() + io.desc() + ">");
+    hipLaunchKernelGGL((mfma_stockham_kernel<DIR, N, W, COL, IO, Rs...>), dim3((unsigned)groups), dim3(TL * W), 0,
+                       st, io, tw);
+    JT_HIP(hipGetLastError());
+}
+
+template <int DIR, bool COL, class IO>
+bool dispatch_mf(int64_t n, IO io, const cpx<float>* tw, hipStream_t st) {
+    switch (n) {
+        case 4096: go_mf<DIR, 4096, COL, IO, 16, 16, 16>(io, tw, st); return true;
+        case 2048: go_mf<DIR, 2048, COL, IO, 16, 16, 8>(io, tw, st); return true;
+        case 1024: go_mf<DIR, 1024, COL, IO, 16, 16, 4>(io, tw, st); return true;
+        case 512: go_mf<DIR, 512, COL, IO, 16, 16, 2>(io, tw, st); return true;
+        case 256: go_mf<DIR, 256, COL, IO, 16, 16>(io, tw, st); return true;
+        case 128: go_mf<DIR, 128, COL, IO, 16, 8>(io, tw, st); return true;
+        case 64: go_mf<DIR, 64, COL, IO, 16, 4>(io, tw, st); return true;
+        case 32: go_mf<DIR, 32, COL, IO, 16, 2>(io, tw, st); return true;
+        case 16: go_mf<DIR, 16, COL, IO, 16>(io, tw, st); return true;
+        default: return false;
+    }
+}
+
 template <typename T, int DIR, bool COL, class IO>
-bool dispatch_tpl(int64_t n, IO io, const cpx<T>* tw, hipStream_t st, bool mf = false) {
+bool dispatch_tpl(int64_t n, IO io, const cpx<T>* tw, hipStream_t st, bool mf = false,
+                  const cpx<T>* tw_mf = nullptr) {
     if constexpr (sizeof(T) == 4 && std::is_same<IO, IOStrided<T>>::value) {
-        if (mf) return dispatch_tpl_mf<T, DIR, COL, true>(n, io, tw, st);
+        if (mf) return dispatch_mf<DIR, COL>(n, io, tw_mf, st);
     }
     return dispatch_tpl_mf<T, DIR, COL, false>(n, io, tw, st);
 }
@@ -395,7 +466,9 @@ void go_gen(const Fft1D<T>& p, IO io, hipStream_t st) {
 template <typename T, int DIR, bool COL, class IO>
 void run_single(const Fft1D<T>& p, IO io, hipStream_t st) {
     if (p.kind == K_TPL) {
-        if (dispatch_tpl<T, DIR, COL>(p.n, io, p.tw.template as<cpx<T>>(), st, p.mf)) return;
+        if (dispatch_tpl<T, DIR, COL>(p.n, io, p.tw.template as<cpx<T>>(), st, p.mf,
+                                      p.tw_mf.template as<cpx<T>>()))
+            return;
     }
     if (p.kind == K_GEN || p.kind == K_TPL) {
         // generic kernel needs a radix plan

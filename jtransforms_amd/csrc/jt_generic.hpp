@@ -30,28 +30,69 @@ __device__ __forceinline__ int fdiv(int a, int d, float rcp) {
     return q;
 }
 
-// Prime radix P (11, 13): direct O(P^2) evaluation; exp(DIR 2 pi i m/P) for
-// m in [0,P) is evaluated once per butterfly.
+// cos / sin (2 pi m / P), m = 0..P-1, for the prime radices 11 and 13
+template <int P>
+struct PrimeTw;
+template <>
+struct PrimeTw<11> {
+    static constexpr double C[11] = {1.0, 0.841253532831181168862, 0.415415013001886425529,
+                                     -0.142314838273285140444, -0.654860733945285064057, -0.95949297361449738989,
+                                     -0.95949297361449738989, -0.654860733945285064057, -0.142314838273285140444,
+                                     0.415415013001886425529, 0.841253532831181168862};
+    static constexpr double S[11] = {0.0, 0.540640817455597582108, 0.909631995354518371412,
+                                     0.989821441880932732376, 0.755749574354258283774, 0.281732556841429697711,
+                                     -0.281732556841429697711, -0.755749574354258283774, -0.989821441880932732376,
+                                     -0.909631995354518371412, -0.540640817455597582108};
+};
+template <>
+struct PrimeTw<13> {
+    static constexpr double C[13] = {1.0, 0.8854560256532098959, 0.568064746731155802512,
+                                     0.120536680255323053349, -0.35460488704253562597, -0.748510748171101098635,
+                                     -0.970941817426052027157, -0.970941817426052027157, -0.748510748171101098635,
+                                     -0.35460488704253562597, 0.120536680255323053349, 0.568064746731155802512,
+                                     0.8854560256532098959};
+    static constexpr double S[13] = {0.0, 0.464723172043768545656, 0.82298386589365639458,
+                                     0.992708874098053992801, 0.93501624268541482344, 0.663122658240795202377,
+                                     0.239315664287557767149, -0.239315664287557767149, -0.663122658240795202377,
+                                     -0.93501624268541482344, -0.992708874098053992801, -0.82298386589365639458,
+                                     -0.464723172043768545656};
+};
+
+// Prime radix P (11, 13), X_k = sum_j v_j exp(DIR 2 pi i jk/P), by pairing
+// j with P-j: with a_j = v_j + v_{P-j}, b_j = v_j - v_{P-j} (j = 1..H,
+// H = (P-1)/2) and m = jk mod P,
+//   X_0     = v_0 + sum a_j
+//   X_k     = v_0 + sum a_j C_m + i DIR sum b_j S_m
+//   X_{P-k} = v_0 + sum a_j C_m - i DIR sum b_j S_m       (k = 1..H)
+// All indices are compile-time constants after unrolling.
 template <int P, int DIR, typename T>
 __device__ __forceinline__ void dft_prime(cpx<T>* v) {
-    cpx<T> w[P];
+    constexpr int H = (P - 1) / 2;
+    cpx<T> a[H + 1], b[H + 1];
+    cpx<T> x0 = v[0];
 #pragma unroll
-    for (int m = 0; m < P; m++) {
-        T s, c;
-        if constexpr (sizeof(T) == 8) sincospi(T(2 * m) / T(P), &s, &c);
-        else sincospif(T(2 * m) / T(P), &s, &c);
-        w[m] = {c, T(DIR) * s};
-    }
-    cpx<T> out[P];
-#pragma unroll
-    for (int k = 0; k < P; k++) {
-        cpx<T> acc = v[0];
-#pragma unroll
-        for (int j = 1; j < P; j++) acc = cadd(acc, cmul(v[j], w[(j * k) % P]));
-        out[k] = acc;
+    for (int j = 1; j <= H; j++) {
+        a[j] = cadd(v[j], v[P - j]);
+        b[j] = csub(v[j], v[P - j]);
+        x0 = cadd(x0, a[j]);
     }
 #pragma unroll
-    for (int k = 0; k < P; k++) v[k] = out[k];
+    for (int k = 1; k <= H; k++) {
+        T re = v[0].x, im = v[0].y, sr = T(0), si = T(0);
+#pragma unroll
+        for (int j = 1; j <= H; j++) {
+            const T c = (T)PrimeTw<P>::C[(j * k) % P];
+            const T sn = (T)PrimeTw<P>::S[(j * k) % P];
+            re += a[j].x * c;
+            im += a[j].y * c;
+            sr += b[j].y * sn;
+            si += b[j].x * sn;
+        }
+        // i DIR (b S) = DIR (-(b.y S), b.x S)
+        v[k] = cpx<T>{re - T(DIR) * sr, im + T(DIR) * si};
+        v[P - k] = cpx<T>{re + T(DIR) * sr, im - T(DIR) * si};
+    }
+    v[0] = x0;
 }
 
 template <int R, int DIR, typename T>

@@ -30,6 +30,10 @@ struct IOStrided {
     bool xcd;
     __device__ __forceinline__ int64_t base(int64_t l) const { return m.base(l); }
     __device__ __forceinline__ cpx<T> load(int64_t b, int k) const { return in[b + k * m.es]; }
+    // one part (0 real, 1 imaginary) of element k
+    __device__ __forceinline__ T loadf(int64_t b, int k, int part) const {
+        return reinterpret_cast<const T*>(in + b + k * m.es)[part];
+    }
     __device__ __forceinline__ void store(int64_t b, int k, cpx<T> v) const {
         out[b + k * m.es] = cpx<T>{v.x * scale, v.y * scale};
     }

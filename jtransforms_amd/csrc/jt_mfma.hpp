@@ -61,6 +61,44 @@ struct MfmaDft16 {
         }
     }
 
+    // One 16-butterfly tile.  b: this lane's 8 entries of the B operand
+    // (column lane%16, rows 8*(lane/16)+u: Re of points 8*((lane/16)&1)+u for
+    // lane/16 < 2, Im of them otherwise).  Returns this lane's 4 outputs
+    // i = 4*(lane/16)+u of butterfly lane%16, real parts in outr, imaginary
+    // parts in outi, in FP32.
+    __device__ __forceinline__ void tile(const float* b, float* outr, float* outi) const {
+        float m = 0.0f;
+#pragma unroll
+        for (int u = 0; u < 8; u++) m = fmaxf(m, fabsf(b[u]));
+        m = fmaxf(m, __shfl_xor(m, 16));
+        m = fmaxf(m, __shfl_xor(m, 32));
+        int ex = 0;
+        (void)frexpf(m, &ex);  // m = f 2^ex, f in [0.5, 1)
+        const int sh = (m > 0.0f) ? (14 - ex) : 0;
+        const float s = ldexpf(1.0f, sh);
+        const float inv = ldexpf(1.0f, -sh);
+        jt_half8 bh, bl;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            float x = b[u] * s;
+            _Float16 h = (_Float16)x;
+            bh[u] = h;
+            bl[u] = (_Float16)(x - (float)h);
+        }
+        jt_float4 dr = {0.f, 0.f, 0.f, 0.f}, di = {0.f, 0.f, 0.f, 0.f};
+        dr = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bh, dr, 0, 0, 0);
+        dr = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bl, dr, 0, 0, 0);
+        dr = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bh, dr, 0, 0, 0);
+        di = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bh, di, 0, 0, 0);
+        di = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bl, di, 0, 0, 0);
+        di = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bh, di, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            outr[u] = dr[u] * inv;
+            outi[u] = di[u] * inv;
+        }
+    }
+
     // v: this lane's butterfly (16 complex, twiddled).  scr: the wave's LDS
     // scratch (MF_WAVE_FLOATS floats).  On return v holds the 16 outputs.
     __device__ __forceinline__ void run(cpx<float>* v, float* scr, int lane) const {

@@ -19,6 +19,7 @@
 // abandoned) so the grid always drains.  Counters are cleared by a kernel, not
 // a memset, so the pair of launches can be captured in a hipGraph.
 #pragma once
+#include <algorithm>
 #include <functional>
 #include <map>
 
@@ -67,7 +68,28 @@ struct RArgs {
     int* err;      // set to 1 if a wait timed out
     int dir;       // -1 forward, +1 backward
     int scaled;    // multiply each transform by 1/n (scaled inverse)
+    int tplx;      // threads per line = (power of two >= n/16) << tplx
     int64_t ntr;
+};
+
+// The tunable schedule of a ragged plan (chosen by ragged_autotune):
+//   wgs    persistent workgroups per CU, i.e. how the CU's LDS is split
+//          (each workgroup holds lines in its share)
+//   nt     tile shape: threads per workgroup (256, 512 or 1024)
+//   tplx   tile shape: threads per line, (power of two >= n/16) << tplx,
+//          so lines per round = nt / threads per line (within the LDS share)
+//   chunk  lines per four-step work item, chunk * (single-kernel limit) points
+//   order  radix order of every runtime plan: 0 = 16, 8/4/2, then odd primes
+//          ascending; 1 = the reverse
+struct RTune {
+    int wgs = 2, nt = 256, tplx = 0, chunk = 4, order = 0;
+    std::string str() const {
+        return "wgs=" + std::to_string(wgs) + ",nt=" + std::to_string(nt) + ",tplx=" + std::to_string(tplx) +
+               ",chunk=" + std::to_string(chunk) + ",order=" + std::to_string(order);
+    }
+    bool operator==(const RTune& o) const {
+        return wgs == o.wgs && nt == o.nt && tplx == o.tplx && chunk == o.chunk && order == o.order;
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -80,16 +102,20 @@ struct RArgs {
 // LDS arrays (each array np_total reals).  load(line, k) / store(line, k, v)
 // move point k of line `line` in [0, cnt).  All threads take every barrier.
 template <typename T, bool LOAD_LF, bool STORE_LF, class LoadF, class StoreF>
-__device__ void rlines(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T* sm, int np_total, int cnt,
-                       LoadF load, StoreF store) {
+__device__ __forceinline__ void rlines(int dir, int tplx, const RLen& L, const cpx<T>* __restrict__ tw, T* sm, int np_total,
+                       int cnt, LoadF load, StoreF store) {
     const int n = L.n;
     const int npl = n + (n >> 5) + 1;
+    const int NT = blockDim.x;
     int tpl = 1;
     while (tpl < 256 && tpl * 16 < n) tpl <<= 1;
-    int G = 256 / tpl;
+    tpl = min(NT, tpl << tplx);
+    int G = NT / tpl;
     const int glds = np_total / npl;
     if (G > glds) G = glds;
     if (G < 1) G = 1;
+    // when LDS caps the line count, the spare threads widen each line
+    while (tpl < NT && 2 * tpl * G <= NT && tpl < n) tpl <<= 1;
     const int t = threadIdx.x;
     const int g = t / tpl, tl = t % tpl;
     T* are = sm + g * npl;
@@ -106,11 +132,11 @@ __device__ void rlines(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T*
             const int nl = min(G, cnt - base);
             const int tot = nl * n;
             const float rnl = 1.0f / (float)nl;
-            for (int i0 = t; i0 < tot; i0 += 4 * 256) {
+            for (int i0 = t; i0 < tot; i0 += 4 * NT) {
                 cpx<T> v[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    const int idx = i0 + u * 256;
+                    const int idx = i0 + u * NT;
                     if (idx < tot) {
                         const int k = fdiv(idx, nl, rnl);
                         v[u] = load(base + idx - k * nl, k);
@@ -118,7 +144,7 @@ __device__ void rlines(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T*
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    const int idx = i0 + u * 256;
+                    const int idx = i0 + u * NT;
                     if (idx < tot) {
                         const int k = fdiv(idx, nl, rnl), lg = idx - k * nl;
                         sm[lg * npl + gpos(k)] = v[u].x;
@@ -164,7 +190,7 @@ __device__ void rlines(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T*
             const int nl = min(G, cnt - base);
             const int o = which ? 2 * np_total : 0;
             const float rnl = 1.0f / (float)nl;
-            for (int idx = t; idx < nl * n; idx += 256) {
+            for (int idx = t; idx < nl * n; idx += NT) {
                 const int k = fdiv(idx, nl, rnl), lg = idx - k * nl;
                 store(base + lg, k, cpx<T>{sm[o + lg * npl + gpos(k)], sm[o + np_total + lg * npl + gpos(k)]});
             }
@@ -179,7 +205,7 @@ __device__ void rlines(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T*
 // FFT of one line held in LDS (a buffers, gpos layout) by the whole
 // workgroup; returns 0 if the result is in a, 1 if in b.
 template <typename T>
-__device__ int rline_fft(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T* are, T* aim, T* bre,
+__device__ __forceinline__ int rline_fft(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T* are, T* aim, T* bre,
                          T* bim, int t, int nt) {
     int ns = 1;
     int which = 0;
@@ -212,7 +238,7 @@ __device__ __forceinline__ cpx<T> rtw2(int dir, const cpx<T>* __restrict__ tw, c
 // condition holds makes the producers' data visible to the whole workgroup
 // (the caches it invalidates are shared by the workgroup's waves).
 // Bounded: after ~2 s of wall clock the wait gives up and flags an error.
-__device__ bool rwait(int* c, int target, int* err) {
+__device__ __forceinline__ bool rwait(int* c, int target, int* err) {
     __shared__ int ok;
     if (threadIdx.x == 0) {
         int good = 1;
@@ -253,8 +279,8 @@ __global__ void ragged_reset(RArgs<T> a) {
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) ragged_kernel(RArgs<T> a, int np) {
+template <typename T, int NTMAX>
+__global__ void __launch_bounds__(NTMAX) ragged_kernel(RArgs<T> a, int np) {
     extern __shared__ unsigned char smem_raw[];
     T* sm = reinterpret_cast<T*>(smem_raw);
     T* are = sm;
@@ -282,9 +308,9 @@ __global__ void __launch_bounds__(256) ragged_kernel(RArgs<T> a, int np) {
                 const int64_t* tl0 = a.tlist + item.c0;
                 const cpx<T>* data = a.data;
                 const int64_t* off = a.off;
-                rlines<T, false, false>(dir, L, a.tw, sm, np, item.cnt,
-                          [&](int line, int k) { return data[off[tl0[line]] + k]; },
-                          [&](int line, int k, cpx<T> v) {
+                rlines<T, false, false>(dir, a.tplx, L, a.tw, sm, np, item.cnt,
+                          [=](int line, int k) { return data[off[tl0[line]] + k]; },
+                          [=](int line, int k, cpx<T> v) {
                               a.data[off[tl0[line]] + k] = cpx<T>{v.x * scale, v.y * scale};
                           });
                 break;
@@ -305,8 +331,8 @@ __global__ void __launch_bounds__(256) ragged_kernel(RArgs<T> a, int np) {
                 if (item.type == RT_BLUE_3 && !rwait(c3 + 1, B.n1, a.err)) break;
                 const int type = item.type, c0 = item.c0, nreal = L.n;
                 const cpx<T>* chirp = a.tw + L.choff;
-                rlines<T, true, true>(d, L1, a.tw, sm, np, item.cnt,
-                          [&](int line, int j1) {
+                rlines<T, true, true>(d, a.tplx, L1, a.tw, sm, np, item.cnt,
+                          [=](int line, int j1) {
                               const int e = j1 * n2 + (c0 + line);
                               if (type == RT_BLUE_1) {
                                   if (e >= nreal) return cpx<T>{T(0), T(0)};
@@ -314,7 +340,7 @@ __global__ void __launch_bounds__(256) ragged_kernel(RArgs<T> a, int np) {
                               }
                               return src[e];
                           },
-                          [&](int line, int k1, cpx<T> v) {
+                          [=](int line, int k1, cpx<T> v) {
                               int j2 = c0 + line;
                               wsA[k1 * n2 + j2] = cmul(v, rtw2<T>(d, a.tw, B, j2 * k1, N));
                           });
@@ -338,9 +364,9 @@ __global__ void __launch_bounds__(256) ragged_kernel(RArgs<T> a, int np) {
                 const int type = item.type, c0 = item.c0, nreal = L.n;
                 const cpx<T>* chirp = a.tw + L.choff;
                 const cpx<T>* bspec = a.tw + L.bsoff;
-                rlines<T, false, true>(d, L2, a.tw, sm, np, item.cnt,
-                          [&](int line, int j2) { return wsA[(c0 + line) * n2 + j2]; },
-                          [&](int line, int k2, cpx<T> v) {
+                rlines<T, false, true>(d, a.tplx, L2, a.tw, sm, np, item.cnt,
+                          [=](int line, int j2) { return wsA[(c0 + line) * n2 + j2]; },
+                          [=](int line, int k2, cpx<T> v) {
                               const int K = (c0 + line) + n1 * k2;
                               if (type == RT_FOUR_B) {
                                   x[K] = cpx<T>{v.x * scale, v.y * scale};
@@ -411,6 +437,7 @@ struct RaggedPlan {
     int grid = 0;
     size_t shm = 0;
     int np = 0;
+    RTune tune;
     // hipGraph cache: one executable graph per (data, stream, dir, scale)
     struct GKey {
         void* data;
@@ -460,10 +487,29 @@ inline std::vector<cpx<double>> bluestein_spectrum(int64_t n, int64_t m, const s
     return b;
 }
 
+// the kernel instantiation for a workgroup size (each has its own register budget)
 template <typename T>
-std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count) {
+const void* ragged_kernel_ptr(int nt) {
+    if (nt == 1024) return (const void*)ragged_kernel<T, 1024>;
+    if (nt == 512) return (const void*)ragged_kernel<T, 512>;
+    if (nt == 256) return (const void*)ragged_kernel<T, 256>;
+    throw JtError(ST_BAD_ARG, "ragged workgroup size must be 256, 512 or 1024");
+}
+
+// Bluestein kernel spectra by length, shared by the candidate plans of a tuning run
+// (chirp, spectrum) per Bluestein length
+using SpecCache = std::map<int64_t, std::pair<std::vector<cpx<double>>, std::vector<cpx<double>>>>;
+
+inline void radix_order(std::vector<int>& r, int order) {
+    if (order == 1) std::reverse(r.begin(), r.end());
+}
+
+template <typename T>
+std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count, const RTune& tune = RTune(),
+                                           SpecCache* spec_cache = nullptr) {
     auto P = std::make_unique<RaggedPlan<T>>();
     P->ntr = count;
+    P->tune = tune;
     // longest line kept in one workgroup's LDS: two ping-pong buffers of this
     // length take ~67 KiB, so two persistent workgroups fit per CU
     const int64_t lim = sizeof(T) == 8 ? 2048 : 4096;
@@ -490,6 +536,7 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
         L.kind = RK_SINGLE;
         std::vector<int> r;
         if (!factor_radices(n, r)) throw JtError(ST_UNSUPPORTED, "internal: non-smooth single length");
+        radix_order(r, tune.order);
         add_twiddles(r, L);
         lens.push_back(L);
         P->maxn = std::max<int>(P->maxn, (int)n);
@@ -528,19 +575,26 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
             int64_t m = next_smooth(2 * n - 1);
             L.m = (int)m;
             L.kind = m <= lim ? RK_BLUE_S : RK_BLUE_L;
-            // chirp exp(i pi (j^2 mod 2n)/n) by the integer recurrence
-            std::vector<cpx<double>> ch(n);
-            const long double PI = 3.141592653589793238462643383279502884L;
-            int64_t k = 0;
-            for (int64_t j = 0; j < n; j++) {
-                if (j) {
-                    k += 2 * j - 1;
-                    if (k >= 2 * n) k -= 2 * n;
+            std::vector<cpx<double>> ch, bs;
+            if (spec_cache && spec_cache->count(n)) {
+                ch = (*spec_cache)[n].first;
+                bs = (*spec_cache)[n].second;
+            } else {
+                // chirp exp(i pi (j^2 mod 2n)/n) by the integer recurrence
+                ch.resize(n);
+                const long double PI = 3.141592653589793238462643383279502884L;
+                int64_t k = 0;
+                for (int64_t j = 0; j < n; j++) {
+                    if (j) {
+                        k += 2 * j - 1;
+                        if (k >= 2 * n) k -= 2 * n;
+                    }
+                    long double ang = PI * (long double)k / (long double)n;
+                    ch[j] = cpx<double>{(double)cosl(ang), (double)sinl(ang)};
                 }
-                long double ang = PI * (long double)k / (long double)n;
-                ch[j] = cpx<double>{(double)cosl(ang), (double)sinl(ang)};
+                bs = bluestein_spectrum(n, m, ch);
+                if (spec_cache) (*spec_cache)[n] = std::make_pair(ch, bs);
             }
-            auto bs = bluestein_spectrum(n, m, ch);
             L.choff = (int64_t)tab.size();
             for (auto& c : ch) tab.push_back(cpx<T>{(T)c.x, (T)c.y});
             L.bsoff = (int64_t)tab.size();
@@ -585,7 +639,7 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
     // work items: cost-balanced phases interleaved with single items
     std::vector<RItem> ph[4], singles;
     // four-step items carry ~4 LDS loads of lines each
-    const int CHUNK = (int)(4 * lim);
+    const int CHUNK = (int)(tune.chunk * lim);
     auto split_items = [&](std::vector<RItem>& v, int type, int len, int64_t tr, int total_lines, int line_len) {
         int per = std::max(1, CHUNK / std::max(1, line_len));
         for (int c = 0; c < total_lines; c += per) v.push_back(RItem{type, len, c, std::min(per, total_lines - c), tr});
@@ -654,16 +708,22 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
 
     // launch shape: LDS for two buffers of the longest line; one persistent
     // workgroup per resident slot
+    // (the LDS is sized to the budget of two workgroups per CU, not to the
+    // longest line: short lines then run many to a workgroup)
     int mx = std::max(P->maxn, 16);
-    P->np = mx + (mx >> 5) + 1;
+    const int need = mx + (mx >> 5) + 1;
+    const int budget = (160 * 1024 - 2048) / std::max(1, tune.wgs) / (4 * (int)sizeof(T));
+    if (budget < need) throw JtError(ST_UNSUPPORTED, "LDS share too small for the longest line");
+    P->np = budget;
     P->shm = 4 * (size_t)P->np * sizeof(T);
-    JT_HIP(hipFuncSetAttribute((const void*)ragged_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)P->shm));
+    const void* kfn = ragged_kernel_ptr<T>(tune.nt);
+    JT_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->shm));
     int dev = 0, cus = 0, per = 0;
     JT_HIP(hipGetDevice(&dev));
     JT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    JT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)ragged_kernel<T>, 256, P->shm));
-    P->grid = std::max(1, cus * std::max(per, 1));
+    JT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, tune.nt, P->shm));
+    if (per < 1) throw JtError(ST_UNSUPPORTED, "ragged schedule does not fit a CU");
+    P->grid = std::max(1, cus * std::max(1, std::min(per, tune.wgs)));
     return P;
 }
 
@@ -684,6 +744,7 @@ RArgs<T> ragged_args(RaggedPlan<T>& P, cpx<T>* data, int dir, bool scaled) {
     a.err = P.d_err.template as<int>();
     a.dir = dir;
     a.scaled = scaled ? 1 : 0;
+    a.tplx = P.tune.tplx;
     a.ntr = P.ntr;
     return a;
 }
@@ -698,8 +759,13 @@ void ragged_launch(RaggedPlan<T>& P, cpx<T>* data, int dir, bool scaled, hipStre
     JT_HIP(hipGetLastError());
     {
         JT_PROF(st, std::string("ragged<") + (sizeof(T) == 8 ? "f64" : "f32") + ",transforms=" +
-                        std::to_string(P.ntr) + ",points=" + std::to_string(P.total) + ">");
-        hipLaunchKernelGGL((ragged_kernel<T>), dim3(P.grid), dim3(256), P.shm, st, a, P.np);
+                        std::to_string(P.ntr) + ",points=" + std::to_string(P.total) + "," + P.tune.str() + ">");
+        if (P.tune.nt == 1024)
+            hipLaunchKernelGGL((ragged_kernel<T, 1024>), dim3(P.grid), dim3(1024), P.shm, st, a, P.np);
+        else if (P.tune.nt == 512)
+            hipLaunchKernelGGL((ragged_kernel<T, 512>), dim3(P.grid), dim3(512), P.shm, st, a, P.np);
+        else
+            hipLaunchKernelGGL((ragged_kernel<T, 256>), dim3(P.grid), dim3(256), P.shm, st, a, P.np);
     }
     JT_HIP(hipGetLastError());
 }
@@ -731,6 +797,101 @@ void ragged_execute(RaggedPlan<T>& P, cpx<T>* data, int dir, bool scale, hipStre
     (void)hipGraphDestroy(graph);
     P.graphs.push_back({typename RaggedPlan<T>::GKey{data, st, dir, scale}, exec});
     JT_HIP(hipGraphLaunch(exec, st));
+}
+
+// ---------------------------------------------------------------------------
+// autotuner
+// ---------------------------------------------------------------------------
+// Coordinate descent over the RTune knobs, timing whole forward launches of
+// candidate plans on a scratch buffer of the plan's size (the running time
+// does not depend on the values).  Each knob is swept with the others held at
+// their best so far; candidates whose LDS share cannot hold the longest line
+// are skipped.  Returns the fastest plan; `log` receives one line per
+// candidate (settings and milliseconds).
+template <typename T>
+std::unique_ptr<RaggedPlan<T>> ragged_autotune(const int64_t* lengths, int64_t count, std::string* log = nullptr) {
+    SpecCache cache;
+    hipStream_t st;
+    JT_HIP(hipStreamCreate(&st));
+    hipEvent_t e0, e1;
+    JT_HIP(hipEventCreate(&e0));
+    JT_HIP(hipEventCreate(&e1));
+    DevBuf scratch;
+    std::unique_ptr<RaggedPlan<T>> best;
+    float best_ms = 0.0f;
+    std::vector<RTune> tried;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(st);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(st);
+    };
+    auto time_plan = [&](RaggedPlan<T>& P) -> float {
+        if (!scratch.p) {
+            scratch.alloc((size_t)std::max<int64_t>(P.total, 1) * sizeof(cpx<T>));
+            JT_HIP(hipMemsetAsync(scratch.p, 0, scratch.bytes, st));
+        }
+        cpx<T>* d = scratch.as<cpx<T>>();
+        ragged_launch(P, d, -1, false, st);  // warm-up
+        JT_HIP(hipEventRecord(e0, st));
+        const int reps = 3;
+        for (int i = 0; i < reps; i++) ragged_launch(P, d, -1, false, st);
+        JT_HIP(hipEventRecord(e1, st));
+        JT_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        JT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        int err = 0;
+        JT_HIP(hipMemcpy(&err, P.d_err.p, sizeof(int), hipMemcpyDeviceToHost));
+        if (err) throw JtError(ST_HIP, "ragged autotune: a dependency wait timed out");
+        return ms / reps;
+    };
+    auto consider = [&](const RTune& t) {
+        for (auto& u : tried)
+            if (u == t) return;
+        tried.push_back(t);
+        std::unique_ptr<RaggedPlan<T>> P;
+        try {
+            P = make_ragged<T>(lengths, count, t, &cache);
+        } catch (const JtError& e) {
+            if (e.code != ST_UNSUPPORTED) throw;
+            if (log) *log += t.str() + " skipped\n";
+            return;
+        }
+        float ms = time_plan(*P);
+        if (log) *log += t.str() + " " + std::to_string(ms) + " ms\n";
+        if (!best || ms < best_ms) {
+            best = std::move(P);
+            best_ms = ms;
+        }
+    };
+    try {
+        consider(RTune());
+        // LDS split and workgroup size together, then the rest one by one
+        const RTune base0 = best->tune;
+        for (int w : {1, 2, 3, 4})
+            for (int nt : {256, 512, 1024}) {
+                RTune t = base0;
+                t.wgs = w;
+                t.nt = nt;
+                if (w * nt <= 2048) consider(t);
+            }
+        for (int knob = 0; knob < 3; knob++) {
+            const std::vector<int> vals = knob == 0 ? std::vector<int>{0, 1, 2}
+                                        : knob == 1 ? std::vector<int>{2, 4, 8, 16, 32}
+                                                    : std::vector<int>{0, 1};
+            const RTune base = best->tune;
+            for (int v : vals) {
+                RTune t = base;
+                (knob == 0 ? t.tplx : knob == 1 ? t.chunk : t.order) = v;
+                consider(t);
+            }
+        }
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+    return best;
 }
 
 }  // namespace jt

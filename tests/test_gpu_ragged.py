@@ -99,3 +99,29 @@ def test_ragged_full_batch(orc, prec):
         worst = max(worst, e / tol(n, prec))
         assert e <= tol(n, prec), (n, e)
     print(f"{prec}: sampled worst error / tolerance = {worst:.3f}", flush=True)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_ragged_autotune(orc, prec):
+    """The autotuned schedule gives the same transforms (to tolerance) and is
+    remembered for the same length list."""
+    import torch
+    lens = np.array(MIXED, dtype=np.int64)
+    p = jt.RaggedBatch1D(lens, precision=prec, autotune=True)
+    d = p.describe()
+    print(d, flush=True)
+    assert "autotuned" in d and "schedule wgs=" in d
+    tot = int(2 * lens.sum())
+    x = orc.jrandom_doubles(20110625, tot).astype(DT[prec])
+    t = torch.from_numpy(x.copy()).cuda()
+    p.complexForward(t)
+    torch.cuda.synchronize()
+    p.sync_status()
+    y = t.cpu().numpy()
+    for i, n in enumerate(lens):
+        seg = slice(int(p.offsets[i]), int(p.offsets[i + 1]))
+        e = rel(y[seg], orc.complex_forward(int(n), x[seg].astype(np.float64)))
+        assert e <= tol(n, prec), (n, e)
+    q = jt.RaggedBatch1D(lens, precision=prec, autotune=True)
+    assert "remembered" in q.describe()
+    assert q.describe().split(" autotuned")[0] == d.split(" autotuned")[0]
