@@ -348,11 +348,31 @@ std::unique_ptr<Fft1D<T>> make_fft1d(int64_t n) {
 // ----------------------------------------------------------------------------
 // launch: compile-time kernels
 // ----------------------------------------------------------------------------
-template <typename T, int N, int TL, bool COL>
+// Column mode reads W adjacent lines per workgroup, i.e. row segments of
+// W * sizeof(cpx<T>) bytes: 128 B when the kernel has no LDS hooks (the
+// exchange then runs in split re/im rounds), 64 B otherwise.
+template <typename T, bool HOOKS>
+constexpr int col_want() {
+    return (HOOKS ? 64 : 128) / (int)sizeof(cpx<T>);
+}
+
+// Threads per line: TL0 for rows; for columns fewer, so that the wanted W
+// lines fit 512 threads, down to 32 points per thread (FP32, and FP64 at
+// N = 4096) or 16 (other FP64 lengths, whose 32-point form spills).
+template <typename T, int N, int TL0, bool COL, bool HOOKS>
+constexpr int line_threads() {
+    if (!COL || HOOKS) return TL0;
+    const int emax = (sizeof(T) == 4 || N == 4096) ? 32 : 16;
+    const int lim = 512 / col_want<T, HOOKS>();
+    const int tl = TL0 < lim ? TL0 : lim;
+    const int tmin = N / emax > 0 ? N / emax : 1;
+    return tl > tmin ? tl : (tmin < TL0 ? tmin : TL0);
+}
+
+template <typename T, int N, int TL, bool COL, bool HOOKS = false>
 struct TplShape {
     static constexpr int lines_row = (256 / TL) > 0 ? (256 / TL) : 1;
-    // column mode: enough adjacent lines for >= 64-byte segments, <= 512 threads
-    static constexpr int want_col = (64 / (int)sizeof(cpx<T>)) > 0 ? 64 / (int)sizeof(cpx<T>) : 1;
+    static constexpr int want_col = col_want<T, HOOKS>() > 0 ? col_want<T, HOOKS>() : 1;
     static constexpr int col_threads_cap = 512 / TL > 0 ? 512 / TL : 1;
     static constexpr int w_col0 = want_col > col_threads_cap ? col_threads_cap : want_col;
     static constexpr int w_col = (w_col0 * TL < 256) ? (256 / TL) : w_col0;
@@ -361,9 +381,13 @@ struct TplShape {
     static constexpr bool split = bytes_full > 65536;
 };
 
-template <typename T, int DIR, int N, int TL, bool COL, bool MF, class IO, int... Rs>
+template <typename T, int DIR, int N, int TL0, bool COL, bool MF, class IO, int... Rs>
 void go_tpl(IO io, const cpx<T>* tw, hipStream_t st) {
-    using S = TplShape<T, N, TL, COL>;
+    // LDS hooks, or a store that computes twiddles (four-step pass A), keep
+    // the narrower shape: their extra registers would spill
+    constexpr bool HOOKS = IO::pre_lds || IO::post_lds || std::is_same<IO, IOFourA<T, DIR>>::value;
+    constexpr int TL = line_threads<T, N, TL0, COL, HOOKS>();
+    using S = TplShape<T, N, TL, COL, HOOKS>;
     constexpr int W = S::W;
     constexpr bool SPLIT = S::split && !IO::pre_lds && !IO::post_lds;
     static_assert(!MF || (W * TL) % 64 == 0, "MFMA stages need whole waves");
@@ -397,7 +421,8 @@ bool dispatch_tpl_mf(int64_t n, IO io, const cpx<T>* tw, hipStream_t st) {
 template <int N, bool COL>
 struct MfShape {
     static constexpr int TL = N / 16;
-    static constexpr int W = COL ? TplShape<float, N, TL, true>::W : (256 / TL > 0 ? 256 / TL : 1);
+    // (columns: 64-B row segments; wider groups measured slower here)
+    static constexpr int W = COL ? TplShape<float, N, TL, true, true>::W : (256 / TL > 0 ? 256 / TL : 1);
 };
 
 template <int DIR, int N, bool COL, class IO, int... Rs>
