@@ -369,10 +369,20 @@ constexpr int line_threads() {
     return tl > tmin ? tl : (tmin < TL0 ? tmin : TL0);
 }
 
-template <typename T, int N, int TL, bool COL, bool HOOKS = false>
+// IO policies over lines of reals (IOR2R) declare real_lines = true
+template <class IO, class = void>
+struct io_real_lines : std::false_type {};
+template <class IO>
+struct io_real_lines<IO, std::void_t<decltype(IO::real_lines)>> : std::integral_constant<bool, IO::real_lines> {};
+
+template <typename T, int N, int TL, bool COL, bool HOOKS = false, bool REAL = false>
 struct TplShape {
     static constexpr int lines_row = (256 / TL) > 0 ? (256 / TL) : 1;
-    static constexpr int want_col = col_want<T, HOOKS>() > 0 ? col_want<T, HOOKS>() : 1;
+    // lines of reals: 256-B row segments while the two LDS arrays fit
+    static constexpr int want_real = (256 / (int)sizeof(T)) * 2 * (N + N / 32 + 32) * (int)sizeof(T) <= 150 * 1024
+                                         ? 256 / (int)sizeof(T)
+                                         : 128 / (int)sizeof(T);
+    static constexpr int want_col = REAL ? want_real : (col_want<T, HOOKS>() > 0 ? col_want<T, HOOKS>() : 1);
     static constexpr int col_threads_cap = 512 / TL > 0 ? 512 / TL : 1;
     static constexpr int w_col0 = want_col > col_threads_cap ? col_threads_cap : want_col;
     static constexpr int w_col = (w_col0 * TL < 256) ? (256 / TL) : w_col0;
@@ -381,15 +391,8 @@ struct TplShape {
     static constexpr bool split = bytes_full > 65536;
 };
 
-template <typename T, int DIR, int N, int TL0, bool COL, bool MF, class IO, int... Rs>
-void go_tpl(IO io, const cpx<T>* tw, hipStream_t st) {
-    // LDS hooks, or a store that computes twiddles (four-step pass A), keep
-    // the narrower shape: their extra registers would spill
-    constexpr bool HOOKS = IO::pre_lds || IO::post_lds || std::is_same<IO, IOFourA<T, DIR>>::value;
-    constexpr int TL = line_threads<T, N, TL0, COL, HOOKS>();
-    using S = TplShape<T, N, TL, COL, HOOKS>;
-    constexpr int W = S::W;
-    constexpr bool SPLIT = S::split && !IO::pre_lds && !IO::post_lds;
+template <typename T, int DIR, int N, int TL, int W, bool COL, bool SPLIT, bool MF, class IO, int... Rs>
+void launch_tpl(IO io, const cpx<T>* tw, hipStream_t st) {
     static_assert(!MF || (W * TL) % 64 == 0, "MFMA stages need whole waves");
     int64_t groups = (io.nlines + W - 1) / W;
     if (groups <= 0) return;
@@ -399,6 +402,28 @@ void go_tpl(IO io, const cpx<T>* tw, hipStream_t st) {
     hipLaunchKernelGGL((stockham_kernel<T, DIR, N, TL, W, COL, SPLIT, MF, IO, Rs...>), dim3((unsigned)groups),
                        dim3(TL * W), 0, st, io, tw);
     JT_HIP(hipGetLastError());
+}
+
+template <typename T, int DIR, int N, int TL0, bool COL, bool MF, class IO, int... Rs>
+void go_tpl(IO io, const cpx<T>* tw, hipStream_t st) {
+    // LDS hooks, or a store that computes twiddles (four-step pass A), keep
+    // the narrower shape: their extra registers would spill
+    constexpr bool HOOKS = IO::pre_lds || IO::post_lds || std::is_same<IO, IOFourA<T, DIR>>::value;
+    constexpr int TL = line_threads<T, N, TL0, COL, HOOKS>();
+    constexpr bool REAL = io_real_lines<IO>::value;
+    using S = TplShape<T, N, TL, COL, HOOKS, false>;
+    constexpr bool SPLIT = S::split && !IO::pre_lds && !IO::post_lds;
+    if constexpr (COL && REAL) {
+        // lines of reals: the wide group (256-B row segments) pays off only
+        // for strides of 1 MiB and more (measured on the 512^3 DCT: 2 MiB
+        // stride 647 -> 568 us, 4 KiB stride 478 -> 546 us)
+        using SW = TplShape<T, N, TL, COL, HOOKS, true>;
+        if (SW::W != S::W && io.m.es * (int64_t)sizeof(T) >= (int64_t(1) << 20)) {
+            launch_tpl<T, DIR, N, TL, SW::W, COL, SPLIT, MF, IO, Rs...>(io, tw, st);
+            return;
+        }
+    }
+    launch_tpl<T, DIR, N, TL, S::W, COL, SPLIT, MF, IO, Rs...>(io, tw, st);
 }
 
 template <typename T, int DIR, bool COL, bool MF, class IO>

@@ -53,6 +53,8 @@ inline R2RScale dct3_scale(int64_t n, bool scale) {
 template <typename T, int KIND, bool FWD>
 struct IOR2R {
     static constexpr bool pre_lds = true, post_lds = true;
+    // lines of reals: a column group's row segment is W * sizeof(T) bytes
+    static constexpr bool real_lines = true;
     static const char* tag() {
         return KIND == R2R_DHT ? "dht" : (KIND == R2R_DCT ? (FWD ? "dct2" : "dct3") : (FWD ? "dst2" : "dst3"));
     }
