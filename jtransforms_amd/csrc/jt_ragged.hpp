@@ -43,7 +43,12 @@ struct RLen {
     int64_t hioff, looff;       // four-step twiddles exp(-2 pi i e / N): hi[e>>8] lo[e&255]
     int m, lm;                  // Bluestein convolution length and its descriptor
     int64_t choff, bsoff;       // chirp (n entries), kernel spectrum (m entries)
+    int tpl;                    // fewest threads per line for the in-place stages
 };
+
+// In-place stages hold at most 16 points per thread.
+template <typename T>
+constexpr int ragged_pts() { return 16; }
 
 struct RItem {
     int type;
@@ -82,7 +87,7 @@ struct RArgs {
 //   order  radix order of every runtime plan: 0 = 16, 8/4/2, then odd primes
 //          ascending; 1 = the reverse
 struct RTune {
-    int wgs = 2, nt = 256, tplx = 0, chunk = 4, order = 0;
+    int wgs = 2, nt = 0, tplx = 0, chunk = 4, order = 0;  // nt 0: smallest that fits
     std::string str() const {
         return "wgs=" + std::to_string(wgs) + ",nt=" + std::to_string(nt) + ",tplx=" + std::to_string(tplx) +
                ",chunk=" + std::to_string(chunk) + ",order=" + std::to_string(order);
@@ -96,32 +101,106 @@ struct RTune {
 // device helpers
 // ---------------------------------------------------------------------------
 
+// One Stockham stage of radix R (span ns) in place on a line's LDS arrays:
+// thread tl of the line's tpl threads gathers the inputs of its butterflies
+// j = tl + b*tpl (b < PM/R; the host sizes tpl so that this covers all n/R)
+// into registers, the workgroup synchronises, and the outputs overwrite the
+// same arrays.  Every thread of the workgroup calls this (the barrier).
+template <int R, int DIR, int PM, typename T>
+__device__ __forceinline__ void stage_ip(T* re, T* im, const cpx<T>* __restrict__ tws, int n, int ns, int tl,
+                                         int tpl, bool act) {
+    constexpr int NB = PM / R;
+    const int m = n / R;
+    cpx<T> v[NB][R];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int j = tl + b * tpl;
+        if (act && j < m) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int k = gpos(j + r * m);
+                v[b][r] = cpx<T>{re[k], im[k]};
+            }
+        }
+    }
+    __syncthreads();
+    const float rns = 1.0f / (float)ns;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int j = tl + b * tpl;
+        if (act && j < m) {
+            const int jh = fdiv(j, ns, rns);
+            const int jm = j - jh * ns;
+            if (ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[b][r] = cmul(v[b][r], tw_dir<DIR>(tws[jm * (R - 1) + r - 1]));
+            }
+            dft_any<R, DIR>(v[b]);
+            const int d = jh * ns * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int k = gpos(d + r * ns);
+                re[k] = v[b][r].x;
+                im[k] = v[b][r].y;
+            }
+        }
+    }
+}
+
+template <int DIR, int PM, typename T>
+__device__ __forceinline__ void stage_ip_rt(int R, T* re, T* im, const cpx<T>* __restrict__ tws, int n, int ns,
+                                            int tl, int tpl, bool act) {
+    switch (R) {
+        case 2: stage_ip<2, DIR, PM>(re, im, tws, n, ns, tl, tpl, act); break;
+        case 3: stage_ip<3, DIR, PM>(re, im, tws, n, ns, tl, tpl, act); break;
+        case 4: stage_ip<4, DIR, PM>(re, im, tws, n, ns, tl, tpl, act); break;
+        case 5: stage_ip<5, DIR, PM>(re, im, tws, n, ns, tl, tpl, act); break;
+        case 7: stage_ip<7, DIR, PM>(re, im, tws, n, ns, tl, tpl, act); break;
+        case 8: stage_ip<8, DIR, PM>(re, im, tws, n, ns, tl, tpl, act); break;
+        case 11: stage_ip<11, DIR, PM>(re, im, tws, n, ns, tl, tpl, act); break;
+        case 13: stage_ip<13, DIR, PM>(re, im, tws, n, ns, tl, tpl, act); break;
+        default: stage_ip<16, DIR, PM>(re, im, tws, n, ns, tl, tpl, act); break;
+    }
+}
+
+// All stages of L in place; the workgroup's threads all call this.
+template <typename T>
+__device__ __forceinline__ void stages_ip(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T* re, T* im,
+                                          int tl, int tpl, bool act) {
+    constexpr int PM = ragged_pts<T>();
+    int ns = 1;
+    for (int s = 0; s < L.nst; s++) {
+        __syncthreads();
+        const int R = L.radix[s];
+        if (dir < 0) stage_ip_rt<-1, PM>(R, re, im, tw + L.twoff[s], L.n, ns, tl, tpl, act);
+        else stage_ip_rt<1, PM>(R, re, im, tw + L.twoff[s], L.n, ns, tl, tpl, act);
+        ns *= R;
+    }
+    __syncthreads();
+}
+
 // Transform `cnt` lines of length L.n held in LDS, G lines at a time: the
-// workgroup's threads split into G groups of tpl threads (tpl ~ the widest
-// stage's butterfly count), each group owning one line's slice of the four
-// LDS arrays (each array np_total reals).  load(line, k) / store(line, k, v)
-// move point k of line `line` in [0, cnt).  All threads take every barrier.
+// workgroup's threads split into G groups of tpl threads, each group owning
+// one line's slice of the two LDS arrays (re, im; each np_total reals).
+// load(line, k) / store(line, k, v) move point k of line `line` in [0, cnt).
+// All threads take every barrier.
 template <typename T, bool LOAD_LF, bool STORE_LF, class LoadF, class StoreF>
-__device__ __forceinline__ void rlines(int dir, int tplx, const RLen& L, const cpx<T>* __restrict__ tw, T* sm, int np_total,
-                       int cnt, LoadF load, StoreF store) {
+__device__ __forceinline__ void rlines(int dir, int tplx, const RLen& L, const cpx<T>* __restrict__ tw, T* sm,
+                                       int np_total, int cnt, LoadF load, StoreF store) {
     const int n = L.n;
     const int npl = n + (n >> 5) + 1;
     const int NT = blockDim.x;
-    int tpl = 1;
-    while (tpl < 256 && tpl * 16 < n) tpl <<= 1;
-    tpl = min(NT, tpl << tplx);
+    int tpl = min(NT, L.tpl << tplx);
     int G = NT / tpl;
     const int glds = np_total / npl;
     if (G > glds) G = glds;
     if (G < 1) G = 1;
     // when LDS caps the line count, the spare threads widen each line
-    while (tpl < NT && 2 * tpl * G <= NT && tpl < n) tpl <<= 1;
+    while (2 * tpl * G <= NT && tpl < n) tpl <<= 1;
     const int t = threadIdx.x;
     const int g = t / tpl, tl = t % tpl;
     T* are = sm + g * npl;
     T* aim = sm + np_total + g * npl;
-    T* bre = sm + 2 * np_total + g * npl;
-    T* bim = sm + 3 * np_total + g * npl;
     for (int base = 0; base < cnt; base += G) {
         const int line = base + g;
         const bool act = g < G && line < cnt;
@@ -170,59 +249,18 @@ __device__ __forceinline__ void rlines(int dir, int tplx, const RLen& L, const c
                 }
             }
         }
-        int ns = 1, which = 0;
-        for (int s = 0; s < L.nst; s++) {
-            __syncthreads();
-            const int R = L.radix[s];
-            if (act) {
-                const T* sr = which ? bre : are;
-                const T* si = which ? bim : aim;
-                T* dr = which ? are : bre;
-                T* di = which ? aim : bim;
-                if (dir < 0) gen_stage_rt<-1>(R, sr, si, dr, di, tw + L.twoff[s], n, ns, tl, tpl);
-                else gen_stage_rt<1>(R, sr, si, dr, di, tw + L.twoff[s], n, ns, tl, tpl);
-            }
-            ns *= R;
-            which ^= 1;
-        }
-        __syncthreads();
+        stages_ip<T>(dir, L, tw, are, aim, tl, tpl, act);
         if constexpr (STORE_LF) {
             const int nl = min(G, cnt - base);
-            const int o = which ? 2 * np_total : 0;
             const float rnl = 1.0f / (float)nl;
             for (int idx = t; idx < nl * n; idx += NT) {
                 const int k = fdiv(idx, nl, rnl), lg = idx - k * nl;
-                store(base + lg, k, cpx<T>{sm[o + lg * npl + gpos(k)], sm[o + np_total + lg * npl + gpos(k)]});
+                store(base + lg, k, cpx<T>{sm[lg * npl + gpos(k)], sm[np_total + lg * npl + gpos(k)]});
             }
         } else if (act) {
-            const T* rr = which ? bre : are;
-            const T* ri = which ? bim : aim;
-            for (int k = tl; k < n; k += tpl) store(line, k, cpx<T>{rr[gpos(k)], ri[gpos(k)]});
+            for (int k = tl; k < n; k += tpl) store(line, k, cpx<T>{are[gpos(k)], aim[gpos(k)]});
         }
     }
-}
-
-// FFT of one line held in LDS (a buffers, gpos layout) by the whole
-// workgroup; returns 0 if the result is in a, 1 if in b.
-template <typename T>
-__device__ __forceinline__ int rline_fft(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T* are, T* aim, T* bre,
-                         T* bim, int t, int nt) {
-    int ns = 1;
-    int which = 0;
-    for (int s = 0; s < L.nst; s++) {
-        __syncthreads();
-        const int R = L.radix[s];
-        const T* sr = which ? bre : are;
-        const T* si = which ? bim : aim;
-        T* dr = which ? are : bre;
-        T* di = which ? aim : bim;
-        if (dir < 0) gen_stage_rt<-1>(R, sr, si, dr, di, tw + L.twoff[s], L.n, ns, t, nt);
-        else gen_stage_rt<1>(R, sr, si, dr, di, tw + L.twoff[s], L.n, ns, t, nt);
-        ns *= R;
-        which ^= 1;
-    }
-    __syncthreads();
-    return which;
 }
 
 template <typename T>
@@ -285,8 +323,6 @@ __global__ void __launch_bounds__(NTMAX) ragged_kernel(RArgs<T> a, int np) {
     T* sm = reinterpret_cast<T*>(smem_raw);
     T* are = sm;
     T* aim = sm + np;
-    T* bre = sm + 2 * np;
-    T* bim = sm + 3 * np;
     __shared__ int cur;
     const int t = threadIdx.x, nt = blockDim.x;
     const int dir = a.dir;
@@ -395,23 +431,17 @@ __global__ void __launch_bounds__(NTMAX) ragged_kernel(RArgs<T> a, int np) {
                     are[gpos(k)] = v.x;
                     aim[gpos(k)] = v.y;
                 }
-                int w = rline_fft<T>(-1, M, a.tw, are, aim, bre, bim, t, nt);
-                T* rr = w ? bre : are;
-                T* ri = w ? bim : aim;
-                T* qr = w ? are : bre;
-                T* qi = w ? aim : bim;
+                stages_ip<T>(-1, M, a.tw, are, aim, t, nt, true);
                 for (int k = t; k < m; k += nt) {
-                    cpx<T> v = {rr[gpos(k)], ri[gpos(k)]};
+                    cpx<T> v = {are[gpos(k)], aim[gpos(k)]};
                     cpx<T> b = a.tw[L.bsoff + k];
                     v = dir < 0 ? cmul(v, b) : cmulc(v, b);
-                    rr[gpos(k)] = v.x;
-                    ri[gpos(k)] = v.y;
+                    are[gpos(k)] = v.x;
+                    aim[gpos(k)] = v.y;
                 }
-                int w2 = rline_fft<T>(1, M, a.tw, rr, ri, qr, qi, t, nt);
-                const T* fr = w2 ? qr : rr;
-                const T* fi = w2 ? qi : ri;
+                stages_ip<T>(1, M, a.tw, are, aim, t, nt, true);
                 for (int k = t; k < n; k += nt) {
-                    cpx<T> v = {fr[gpos(k)], fi[gpos(k)]};
+                    cpx<T> v = {are[gpos(k)], aim[gpos(k)]};
                     cpx<T> c = a.tw[L.choff + k];
                     v = dir < 0 ? cmulc(v, c) : cmul(v, c);
                     x[k] = cpx<T>{v.x * scale, v.y * scale};
@@ -510,11 +540,11 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
     auto P = std::make_unique<RaggedPlan<T>>();
     P->ntr = count;
     P->tune = tune;
-    // longest line kept in one workgroup's LDS: two ping-pong buffers of this
-    // length take ~67 KiB, so two persistent workgroups fit per CU
+    // longest line kept in one workgroup's LDS (re and im arrays)
     const int64_t lim = sizeof(T) == 8 ? 2048 : 4096;
     std::vector<RLen> lens;
     std::vector<cpx<T>> tab;
+    int max_tpl = 1;                     // largest threads-per-line requirement
     std::map<int64_t, int> idx_single;  // single-kernel descriptor by n
     std::map<int64_t, int> idx_len;     // top-level descriptor by n
     auto add_twiddles = [&](const std::vector<int>& r, RLen& L) {
@@ -538,6 +568,15 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
         if (!factor_radices(n, r)) throw JtError(ST_UNSUPPORTED, "internal: non-smooth single length");
         radix_order(r, tune.order);
         add_twiddles(r, L);
+        // threads per line so that every stage's butterflies fit the
+        // in-place stage's register budget (ragged_pts points per thread)
+        L.tpl = 1;
+        for (int R : r) {
+            const int per = std::max(1, ragged_pts<T>() / R);  // butterflies per thread
+            const int m = (int)(n / R);
+            L.tpl = std::max(L.tpl, (m + per - 1) / per);
+        }
+        max_tpl = std::max(max_tpl, L.tpl);
         lens.push_back(L);
         P->maxn = std::max<int>(P->maxn, (int)n);
         return idx_single[n] = (int)lens.size() - 1;
@@ -712,16 +751,21 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
     // longest line: short lines then run many to a workgroup)
     int mx = std::max(P->maxn, 16);
     const int need = mx + (mx >> 5) + 1;
-    const int budget = (160 * 1024 - 2048) / std::max(1, tune.wgs) / (4 * (int)sizeof(T));
+    if (P->tune.nt == 0) {
+        P->tune.nt = 256;
+        while (P->tune.nt < max_tpl && P->tune.nt < 1024) P->tune.nt *= 2;
+    }
+    if (max_tpl > P->tune.nt) throw JtError(ST_UNSUPPORTED, "workgroup too small for the longest line");
+    const int budget = (160 * 1024 - 2048) / std::max(1, tune.wgs) / (2 * (int)sizeof(T));
     if (budget < need) throw JtError(ST_UNSUPPORTED, "LDS share too small for the longest line");
     P->np = budget;
-    P->shm = 4 * (size_t)P->np * sizeof(T);
-    const void* kfn = ragged_kernel_ptr<T>(tune.nt);
+    P->shm = 2 * (size_t)P->np * sizeof(T);
+    const void* kfn = ragged_kernel_ptr<T>(P->tune.nt);
     JT_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->shm));
     int dev = 0, cus = 0, per = 0;
     JT_HIP(hipGetDevice(&dev));
     JT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    JT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, tune.nt, P->shm));
+    JT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, P->tune.nt, P->shm));
     if (per < 1) throw JtError(ST_UNSUPPORTED, "ragged schedule does not fit a CU");
     P->grid = std::max(1, cus * std::max(1, std::min(per, tune.wgs)));
     return P;
@@ -857,8 +901,13 @@ std::unique_ptr<RaggedPlan<T>> ragged_autotune(const int64_t* lengths, int64_t c
             if (log) *log += t.str() + " skipped\n";
             return;
         }
+        if (!(P->tune == t)) {  // an automatic setting resolved to a concrete one
+            for (auto& u : tried)
+                if (u == P->tune) return;
+            tried.push_back(P->tune);
+        }
         float ms = time_plan(*P);
-        if (log) *log += t.str() + " " + std::to_string(ms) + " ms\n";
+        if (log) *log += P->tune.str() + " " + std::to_string(ms) + " ms\n";
         if (!best || ms < best_ms) {
             best = std::move(P);
             best_ms = ms;
