@@ -541,7 +541,7 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
     P->ntr = count;
     P->tune = tune;
     // longest line kept in one workgroup's LDS (re and im arrays)
-    const int64_t lim = sizeof(T) == 8 ? 2048 : 4096;
+    const int64_t lim = sizeof(T) == 8 ? 4096 : 8192;
     std::vector<RLen> lens;
     std::vector<cpx<T>> tab;
     int max_tpl = 1;                     // largest threads-per-line requirement

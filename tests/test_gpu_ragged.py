@@ -25,7 +25,7 @@ def rel(a, b):
 
 
 DT = {"f64": np.float64, "f32": np.float32}
-# every item kind: LDS singles, four-step (n > 2048 FP64 / 4096 FP32),
+# every item kind: LDS singles, four-step (n > 4096 FP64 / 8192 FP32),
 # Bluestein in LDS and Bluestein over a four-step convolution length
 MIXED = [1, 2, 3, 7, 16, 17, 100, 211, 1000, 1024, 2048, 2049, 3000, 4096, 5000, 8192, 10007, 12800,
          65536, 65521, 4093, 240, 6561]
