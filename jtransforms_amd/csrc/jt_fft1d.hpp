@@ -391,20 +391,19 @@ struct TplShape {
     static constexpr bool split = bytes_full > 65536;
 };
 
-template <typename T, int DIR, int N, int TL, int W, bool COL, bool SPLIT, bool MF, class IO, int... Rs>
+template <typename T, int DIR, int N, int TL, int W, bool COL, bool SPLIT, class IO, int... Rs>
 void launch_tpl(IO io, const cpx<T>* tw, hipStream_t st) {
-    static_assert(!MF || (W * TL) % 64 == 0, "MFMA stages need whole waves");
     int64_t groups = (io.nlines + W - 1) / W;
     if (groups <= 0) return;
     JT_PROF(st, std::string("stockham<") + (sizeof(T) == 8 ? "f64" : "f32") + ",N=" + std::to_string(N) +
- (COL ? ",col" : ",row") + (MF ? ",mfma" : "") + ",W=" + std::to_string(W) + ",lines=" + std::to_string(io.nlines) +
+ (COL ? ",col" : ",row") + ",W=" + std::to_string(W) + ",lines=" + std::to_string(io.nlines) +
                     "," + IO::tag() + io.desc() + ">");
-    hipLaunchKernelGGL((stockham_kernel<T, DIR, N, TL, W, COL, SPLIT, MF, IO, Rs...>), dim3((unsigned)groups),
+    hipLaunchKernelGGL((stockham_kernel<T, DIR, N, TL, W, COL, SPLIT, IO, Rs...>), dim3((unsigned)groups),
                        dim3(TL * W), 0, st, io, tw);
     JT_HIP(hipGetLastError());
 }
 
-template <typename T, int DIR, int N, int TL0, bool COL, bool MF, class IO, int... Rs>
+template <typename T, int DIR, int N, int TL0, bool COL, class IO, int... Rs>
 void go_tpl(IO io, const cpx<T>* tw, hipStream_t st) {
     // LDS hooks, or a store that computes twiddles (four-step pass A), keep
     // the narrower shape: their extra registers would spill
@@ -419,25 +418,25 @@ void go_tpl(IO io, const cpx<T>* tw, hipStream_t st) {
         // stride 647 -> 568 us, 4 KiB stride 478 -> 546 us)
         using SW = TplShape<T, N, TL, COL, HOOKS, true>;
         if (SW::W != S::W && io.m.es * (int64_t)sizeof(T) >= (int64_t(1) << 20)) {
-            launch_tpl<T, DIR, N, TL, SW::W, COL, SPLIT, MF, IO, Rs...>(io, tw, st);
+            launch_tpl<T, DIR, N, TL, SW::W, COL, SPLIT, IO, Rs...>(io, tw, st);
             return;
         }
     }
-    launch_tpl<T, DIR, N, TL, S::W, COL, SPLIT, MF, IO, Rs...>(io, tw, st);
+    launch_tpl<T, DIR, N, TL, S::W, COL, SPLIT, IO, Rs...>(io, tw, st);
 }
 
-template <typename T, int DIR, bool COL, bool MF, class IO>
-bool dispatch_tpl_mf(int64_t n, IO io, const cpx<T>* tw, hipStream_t st) {
+template <typename T, int DIR, bool COL, class IO>
+bool dispatch_tpl_n(int64_t n, IO io, const cpx<T>* tw, hipStream_t st) {
     switch (n) {
-        case 4096: go_tpl<T, DIR, 4096, 256, COL, MF, IO, 16, 16, 16>(io, tw, st); return true;
-        case 2048: go_tpl<T, DIR, 2048, 128, COL, MF, IO, 16, 16, 8>(io, tw, st); return true;
-        case 1024: go_tpl<T, DIR, 1024, 64, COL, MF, IO, 16, 16, 4>(io, tw, st); return true;
-        case 512: go_tpl<T, DIR, 512, 64, COL, false, IO, 8, 8, 8>(io, tw, st); return true;
-        case 256: go_tpl<T, DIR, 256, 16, COL, MF, IO, 16, 16>(io, tw, st); return true;
-        case 128: go_tpl<T, DIR, 128, 8, COL, false, IO, 16, 8>(io, tw, st); return true;
-        case 64: go_tpl<T, DIR, 64, 8, COL, false, IO, 8, 8>(io, tw, st); return true;
-        case 32: go_tpl<T, DIR, 32, 4, COL, false, IO, 8, 4>(io, tw, st); return true;
-        case 16: go_tpl<T, DIR, 16, 1, COL, MF, IO, 16>(io, tw, st); return true;
+        case 4096: go_tpl<T, DIR, 4096, 256, COL, IO, 16, 16, 16>(io, tw, st); return true;
+        case 2048: go_tpl<T, DIR, 2048, 128, COL, IO, 16, 16, 8>(io, tw, st); return true;
+        case 1024: go_tpl<T, DIR, 1024, 64, COL, IO, 16, 16, 4>(io, tw, st); return true;
+        case 512: go_tpl<T, DIR, 512, 64, COL, IO, 8, 8, 8>(io, tw, st); return true;
+        case 256: go_tpl<T, DIR, 256, 16, COL, IO, 16, 16>(io, tw, st); return true;
+        case 128: go_tpl<T, DIR, 128, 8, COL, IO, 16, 8>(io, tw, st); return true;
+        case 64: go_tpl<T, DIR, 64, 8, COL, IO, 8, 8>(io, tw, st); return true;
+        case 32: go_tpl<T, DIR, 32, 4, COL, IO, 8, 4>(io, tw, st); return true;
+        case 16: go_tpl<T, DIR, 16, 1, COL, IO, 16>(io, tw, st); return true;
         default: return false;
     }
 }
@@ -485,7 +484,7 @@ bool dispatch_tpl(int64_t n, IO io, const cpx<T>* tw, hipStream_t st, bool mf = 
     if constexpr (sizeof(T) == 4 && std::is_same<IO, IOStrided<T>>::value) {
         if (mf) return dispatch_mf<DIR, COL>(n, io, tw_mf, st);
     }
-    return dispatch_tpl_mf<T, DIR, COL, false>(n, io, tw, st);
+    return dispatch_tpl_n<T, DIR, COL>(n, io, tw, st);
 }
 
 template <typename T, int DIR, class IO>

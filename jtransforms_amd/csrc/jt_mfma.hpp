@@ -1,8 +1,8 @@
 // Radix-16 butterflies as a dense DFT-matrix product on MFMA (gfx950), in
 // split FP16 with FP32 accumulation.
 //
-// A wave holds 64 butterflies (one per lane, 16 complex inputs each, twiddles
-// already applied).  They are processed as 4 tiles of 16 butterflies:
+// A wave processes 64 butterflies (16 complex inputs each, twiddles already
+// applied) as 4 tiles of 16 butterflies:
 //     X = F16 * V,  F16[i][k] = exp(DIR 2 pi i ik/16)
 // written over the reals as two 16x32 by 32x16 products:
 //     Xr = [Fr | -Fi] [Vr; Vi],   Xi = [Fi | Fr] [Vr; Vi]
@@ -24,10 +24,6 @@ namespace jt {
 
 typedef _Float16 jt_half8 __attribute__((ext_vector_type(8)));
 typedef float jt_float4 __attribute__((ext_vector_type(4)));
-
-// LDS scratch per wave: 64 butterflies x 16 points, re and im, row pitch 17
-constexpr int MF_PITCH = 17;
-constexpr int MF_WAVE_FLOATS = 2 * 64 * MF_PITCH;
 
 struct MfmaDft16 {
     jt_half8 arh, arl, aih, ail;  // A operands of Xr and Xi, hi/lo
@@ -97,82 +93,6 @@ struct MfmaDft16 {
             outr[u] = dr[u] * inv;
             outi[u] = di[u] * inv;
         }
-    }
-
-    // v: this lane's butterfly (16 complex, twiddled).  scr: the wave's LDS
-    // scratch (MF_WAVE_FLOATS floats).  On return v holds the 16 outputs.
-    __device__ __forceinline__ void run(cpx<float>* v, float* scr, int lane) const {
-        float* sre = scr;
-        float* sim = scr + 64 * MF_PITCH;
-        // 1. park the wave's butterflies in LDS, [butterfly][point]
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            sre[lane * MF_PITCH + r] = v[r].x;
-            sim[lane * MF_PITCH + r] = v[r].y;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // 2. gather the B operands of the 4 tiles: lane l, tile q holds
-        //    column j = l%16 (butterfly 16q+j), rows k = 8(l/16)+v:
-        //    l/16 in {0,1}: Re of points 8(l/16 & 1)+v; {2,3}: Im of them
-        const int j = lane % 16;
-        const int g = lane / 16;
-        const float* src = (g >= 2) ? sim : sre;
-        const int r0 = 8 * (g & 1);
-        float b[4][8];
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-#pragma unroll
-            for (int u = 0; u < 8; u++) b[q][u] = src[(16 * q + j) * MF_PITCH + r0 + u];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // 3. per tile: scale, split, 6 MFMAs, unscale, park outputs
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            float m = 0.0f;
-#pragma unroll
-            for (int u = 0; u < 8; u++) m = fmaxf(m, fabsf(b[q][u]));
-            m = fmaxf(m, __shfl_xor(m, 16));
-            m = fmaxf(m, __shfl_xor(m, 32));
-            int ex = 0;
-            (void)frexpf(m, &ex);  // m = f 2^ex, f in [0.5, 1)
-            const int sh = (m > 0.0f) ? (14 - ex) : 0;
-            const float s = ldexpf(1.0f, sh);
-            const float inv = ldexpf(1.0f, -sh);
-            jt_half8 bh, bl;
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                float x = b[q][u] * s;
-                _Float16 h = (_Float16)x;
-                bh[u] = h;
-                bl[u] = (_Float16)(x - (float)h);
-            }
-            jt_float4 dr = {0.f, 0.f, 0.f, 0.f}, di = {0.f, 0.f, 0.f, 0.f};
-            dr = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bh, dr, 0, 0, 0);
-            dr = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bl, dr, 0, 0, 0);
-            dr = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bh, dr, 0, 0, 0);
-            di = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bh, di, 0, 0, 0);
-            di = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bl, di, 0, 0, 0);
-            di = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bh, di, 0, 0, 0);
-            // D[i][j], i = 4*(l/16)+v: output i of butterfly 16q+j
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                int i = 4 * g + u;
-                sre[(16 * q + j) * MF_PITCH + i] = dr[u] * inv;
-                sim[(16 * q + j) * MF_PITCH + i] = di[u] * inv;
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // 4. each lane takes back its butterfly's outputs
-#pragma unroll
-        for (int r = 0; r < 16; r++) v[r] = cpx<float>{sre[lane * MF_PITCH + r], sim[lane * MF_PITCH + r]};
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 };
 

@@ -17,7 +17,6 @@
 // exchanged in two rounds so the buffer is half the size (more waves per CU).
 #pragma once
 #include "jt_dft.hpp"
-#include "jt_mfma.hpp"
 
 namespace jt {
 
@@ -161,25 +160,16 @@ struct Engine {
     // Run stages S.. (radix R, span NS) given that v holds the *inputs* of
     // stage S (already exchanged).  Returns with v holding the outputs of the
     // last stage, butterfly-major: v[q*RL + r] is element t + TL*(q + r*E/RL).
-    // MF: radix-16 stages run as split-FP16 MFMA products (T = float, one
-    // radix-16 butterfly per thread); mf/scr/lane describe the wave's MFMA
-    // operands and LDS scratch.
-    template <bool MF, class List, int S, int NS, int R, int... Rest>
+    template <class List, int S, int NS, int R, int... Rest>
     static __device__ __forceinline__ void stages(LdsBuf<T, N, W, SPLIT> lds, int w, int t,
-                                                  const cpx<T>* __restrict__ tw, cpx<T>* v,
-                                                  const MfmaDft16* mf, float* scr, int lane) {
+                                                  const cpx<T>* __restrict__ tw, cpx<T>* v) {
         twiddle<R, NS, List::twoff(S)>(tw, t, v);
-        if constexpr (MF && R == 16) {
-            static_assert(E == 16, "MFMA radix-16 stages need one butterfly per thread");
-            mf->run(reinterpret_cast<cpx<float>*>(v), scr, lane);
-        } else {
-            butterflies<R>(v);
-        }
+        butterflies<R>(v);
         if constexpr (sizeof...(Rest) > 0) {
             constexpr int RN = first<Rest...>();
             static_assert(E % RN == 0, "radix must divide the per-thread point count");
             exchange<R, NS, RN>(lds, w, t, v);
-            stages<MF, List, S + 1, NS * R, Rest...>(lds, w, t, tw, v, mf, scr, lane);
+            stages<List, S + 1, NS * R, Rest...>(lds, w, t, tw, v);
         }
     }
 
@@ -226,10 +216,8 @@ __device__ __forceinline__ int64_t xcd_group(int64_t bid, int64_t ngroups) {
 //   post_lds: __device__ void post(int64_t line, int64_t base, const T* re,
 //             const T* im, int t) consumes the N outputs from LDS.
 // ----------------------------------------------------------------------------
-template <typename T, int DIR, int N, int TL, int W, bool COLMODE, bool SPLIT, bool MF, class IO,
-          int... Rs>
+template <typename T, int DIR, int N, int TL, int W, bool COLMODE, bool SPLIT, class IO, int... Rs>
 __global__ void __launch_bounds__(TL* W) stockham_kernel(IO io, const cpx<T>* __restrict__ tw) {
-    static_assert(!MF || sizeof(T) == 4, "split-FP16 MFMA stages are an FP32 form");
     using Eng = Engine<T, DIR, N, TL, W, COLMODE, SPLIT>;
     using G = LdsGeom<N, W>;
     constexpr int E = N / TL;
@@ -239,12 +227,7 @@ __global__ void __launch_bounds__(TL* W) stockham_kernel(IO io, const cpx<T>* __
     constexpr bool need_im = (!SPLIT) || IO::pre_lds || IO::post_lds;
     __shared__ T lds_re[G::size];
     __shared__ T lds_im[need_im ? G::size : 1];
-    constexpr int NWAVES = (TL * W + 63) / 64;
-    __shared__ float mf_scr[MF ? NWAVES * MF_WAVE_FLOATS : 1];
     LdsBuf<T, N, W, SPLIT> lds{lds_re, lds_im};
-    MfmaDft16 mf;
-    const int lane = threadIdx.x & 63;
-    if constexpr (MF) mf.template init<DIR>(lane);
 
     const int tid = threadIdx.x;
     const int w = COLMODE ? tid % W : tid / TL;
@@ -272,9 +255,7 @@ __global__ void __launch_bounds__(TL* W) stockham_kernel(IO io, const cpx<T>* __
             }
     }
 
-    Eng::template stages<MF, RadixList<Rs...>, 0, 1, Rs...>(lds, w, t, tw, v, &mf,
-                                                           mf_scr + (threadIdx.x / 64) * (MF ? MF_WAVE_FLOATS : 0),
-                                                           lane);
+    Eng::template stages<RadixList<Rs...>, 0, 1, Rs...>(lds, w, t, tw, v);
 
     if constexpr (IO::post_lds) {
         Eng::template put_final<RL, NL>(lds, w, t, v);
