@@ -131,28 +131,32 @@ struct IOR2C {
     __device__ void pre(int64_t, int64_t, T*, T*, int, int) const {}
     template <int CN, int CTL>
     __device__ void post(int64_t, int64_t b, const T* re, const T* im, int t, int nt) const {
-        T* o = out + 2 * b;
+        // a[2k], a[2k+1] = Re, Im X_k are one complex store at o[k]
+        cpx<T>* o = reinterpret_cast<cpx<T>*>(out + 2 * b);
         const T h = T(0.5) * scale;
-        for (int k = t; k <= N / 2; k += nt) {
-            if (k == 0) {
-                T zr = re[lpos(0)], zi = im[lpos(0)];
-                o[0] = (zr + zi) * scale;
-                o[1] = (zr - zi) * scale;
-                continue;
-            }
-            int kk = N - k;
-            cpx<T> A = {re[lpos(k)], im[lpos(k)]};
-            cpx<T> B = {re[lpos(kk)], -im[lpos(kk)]};  // conj Z_{N-k}
-            cpx<T> fe = cadd(A, B);
-            cpx<T> d = csub(A, B);
-            cpx<T> fo = {d.y, -d.x};  // -i * d
-            cpx<T> wf = cmul(rtw[k], fo);
-            // X_k = (fe + w^k fo)/2 ; X_{N-k} = conj(fe - w^k fo)/2
-            o[2 * k] = (fe.x + wf.x) * h;
-            o[2 * k + 1] = (fe.y + wf.y) * h;
-            if (kk != k) {
-                o[2 * kk] = (fe.x - wf.x) * h;
-                o[2 * kk + 1] = -(fe.y - wf.y) * h;
+        if (t == 0) {
+            T zr = re[lpos(0)], zi = im[lpos(0)];
+            o[0] = cpx<T>{(zr + zi) * scale, (zr - zi) * scale};
+        }
+        const int n = CN > 0 ? CN : N;
+        const int step = CTL > 0 ? CTL : nt;
+        // k = 1 .. n/2, with compile-time trip counts when CN, CTL are known
+        constexpr int ITER = (CN > 0 && CTL > 0) ? (CN / 2 + CTL - 1) / CTL : 1;
+        for (int k0 = 1 + t; k0 <= n / 2; k0 += ITER * step) {
+#pragma unroll
+            for (int i = 0; i < ITER; i++) {
+                const int k = k0 + i * step;
+                if (k > n / 2) break;
+                const int kk = n - k;
+                cpx<T> A = {re[lpos(k)], im[lpos(k)]};
+                cpx<T> B = {re[lpos(kk)], -im[lpos(kk)]};  // conj Z_{N-k}
+                cpx<T> fe = cadd(A, B);
+                cpx<T> d = csub(A, B);
+                cpx<T> fo = {d.y, -d.x};  // -i * d
+                cpx<T> wf = cmul(rtw[k], fo);
+                // X_k = (fe + w^k fo)/2 ; X_{N-k} = conj(fe - w^k fo)/2
+                o[k] = cpx<T>{(fe.x + wf.x) * h, (fe.y + wf.y) * h};
+                if (kk != k) o[kk] = cpx<T>{(fe.x - wf.x) * h, -(fe.y - wf.y) * h};
             }
         }
     }
