@@ -200,6 +200,9 @@ void run_complex_nd(jt_plan* p, cpx<T>* a, bool scale, hipStream_t st) {
         c2c_contig<T, DIR>(*I.ax[0], a, a, B, s, ws, st);
         return;
     }
+    // (no streaming loads or stores here: the passes after the first read
+    // what the previous pass left in the caches, and streaming the first
+    // pass's loads measured slower overall: 4096^2 FP64 0.42 -> 0.48 ms)
     if (nd == 2) {
         const int64_t r = p->dims[0], c = p->dims[1];
         axis_c2c<T, DIR>(*I.ax[1], a, LineMap{B * r, B * r, c, 0, 1}, T(1), ws, st);

@@ -441,6 +441,11 @@ bool dispatch_tpl_n(int64_t n, IO io, const cpx<T>* tw, hipStream_t st) {
     }
 }
 
+template <class IO>
+struct is_strided : std::false_type {};
+template <typename T, int NT>
+struct is_strided<IOStrided<T, NT>> : std::true_type {};
+
 // split-FP16 MFMA kernels: W lines per workgroup, N/16 threads per line
 template <int N, bool COL>
 struct MfShape {
@@ -481,7 +486,7 @@ bool dispatch_mf(int64_t n, IO io, const cpx<float>* tw, hipStream_t st) {
 template <typename T, int DIR, bool COL, class IO>
 bool dispatch_tpl(int64_t n, IO io, const cpx<T>* tw, hipStream_t st, bool mf = false,
                   const cpx<T>* tw_mf = nullptr) {
-    if constexpr (sizeof(T) == 4 && std::is_same<IO, IOStrided<T>>::value) {
+    if constexpr (sizeof(T) == 4 && is_strided<IO>::value) {
         if (mf) return dispatch_mf<DIR, COL>(n, io, tw_mf, st);
     }
     return dispatch_tpl_n<T, DIR, COL>(n, io, tw, st);
@@ -592,12 +597,27 @@ template <typename T, int DIR>
 void c2c_contig(const Fft1D<T>& p, const cpx<T>* in, cpx<T>* out, int64_t lines, T scale,
                 cpx<T>* ws, hipStream_t st);
 
+// arrays at least this large are streamed (non-temporal) by first/last passes
+constexpr int64_t NT_MIN_BYTES = int64_t(64) << 20;
+
 template <typename T, int DIR>
 void c2c_mapped(const Fft1D<T>& p, const cpx<T>* in, cpx<T>* out, LineMap lm, T scale,
-                hipStream_t st) {
-    IOStrided<T> io{in, out, lm, lm.nlines, scale, lm.es != 1};
-    if (lm.es == 1) run_single<T, DIR, false>(p, io, st);
-    else run_single<T, DIR, true>(p, io, st);
+                hipStream_t st, int nt = 0) {
+    // streaming (nt = NT_LOAD | NT_STORE) only for a single pass over
+    // contiguous lines, i.e. a batch of 1D transforms whose result is not
+    // read by a following pass: 16384 x 4096 FP32 218 -> 208 us.  In column
+    // mode a workgroup stores partial row segments, and non-temporal stores
+    // of those measured twice as slow.
+    if (lm.es != 1) {
+        IOStrided<T> io{in, out, lm, lm.nlines, scale, true};
+        run_single<T, DIR, true>(p, io, st);
+    } else if (nt == (NT_LOAD | NT_STORE)) {
+        IOStrided<T, NT_LOAD | NT_STORE> io{in, out, lm, lm.nlines, scale, false};
+        run_single<T, DIR, false>(p, io, st);
+    } else {
+        IOStrided<T> io{in, out, lm, lm.nlines, scale, false};
+        run_single<T, DIR, false>(p, io, st);
+    }
 }
 
 template <typename T, int DIR>
@@ -606,7 +626,9 @@ void c2c_contig(const Fft1D<T>& p, const cpx<T>* in, cpx<T>* out, int64_t lines,
     if (lines <= 0) return;
     if (p.single()) {
         LineMap lm{lines, lines, p.n, 0, 1};
-        c2c_mapped<T, DIR>(p, in, out, lm, scale, st);
+        // a single pass over a large batch reads and writes each element once
+        const bool big = lines * p.n * (int64_t)sizeof(cpx<T>) >= NT_MIN_BYTES;
+        c2c_mapped<T, DIR>(p, in, out, lm, scale, st, big ? (NT_LOAD | NT_STORE) : 0);
         return;
     }
     if (p.kind == K_FOUR) {
