@@ -39,8 +39,11 @@ class ProfScope {
         if (!prof_on().load(std::memory_order_relaxed)) return;
         active_ = true;
         name_ = fn();
-        (void)hipEventCreate(&a_);
-        (void)hipEventCreate(&b_);
+        // timing-only events: no system-scope fence (cache writeback and
+        // invalidate) when recorded, so the measured kernels run as they
+        // would unprofiled
+        (void)hipEventCreateWithFlags(&a_, hipEventDisableSystemFence);
+        (void)hipEventCreateWithFlags(&b_, hipEventDisableSystemFence);
         (void)hipEventRecord(a_, st_);
     }
     ~ProfScope() {
