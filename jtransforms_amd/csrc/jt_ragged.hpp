@@ -20,6 +20,7 @@
 // a memset, so the pair of launches can be captured in a hipGraph.
 #pragma once
 #include <algorithm>
+#include <cstdlib>
 #include <functional>
 #include <map>
 
@@ -74,6 +75,8 @@ struct RArgs {
     int dir;       // -1 forward, +1 backward
     int scaled;    // multiply each transform by 1/n (scaled inverse)
     int tplx;      // threads per line = (power of two >= n/16) << tplx
+    int wait_extra;          // 0; 1 makes every dependency target unreachable (failure-path test)
+    uint64_t wait_ticks;     // bound of one wait, in 100 MHz wall-clock ticks
     int64_t ntr;
 };
 
@@ -275,15 +278,16 @@ __device__ __forceinline__ cpx<T> rtw2(int dir, const cpx<T>* __restrict__ tw, c
 // Polls are relaxed; one agent-scope acquire fence by thread 0 after the
 // condition holds makes the producers' data visible to the whole workgroup
 // (the caches it invalidates are shared by the workgroup's waves).
-// Bounded: after ~2 s of wall clock the wait gives up and flags an error.
-__device__ __forceinline__ bool rwait(int* c, int target, int* err) {
+// Bounded: after `ticks` of wall clock (2 s normally) the wait gives up and
+// flags an error.
+__device__ __forceinline__ bool rwait(int* c, int target, int* err, uint64_t ticks) {
     __shared__ int ok;
     if (threadIdx.x == 0) {
         int good = 1;
         uint64_t t0 = wall_clock64();
         while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(8);
-            if (wall_clock64() - t0 > 200000000ull) {  // 100 MHz counter: 2 s
+            if (wall_clock64() - t0 > ticks) {  // 100 MHz counter
                 good = 0;
                 atomicExch(err, 1);
                 break;
@@ -364,7 +368,7 @@ __global__ void __launch_bounds__(NTMAX) ragged_kernel(RArgs<T> a, int np) {
                 const int d = item.type == RT_BLUE_1 ? -1 : (item.type == RT_BLUE_3 ? 1 : dir);
                 cpx<T>* wsA = a.ws + a.wsoff[tr];
                 const cpx<T>* src = item.type == RT_BLUE_3 ? wsA + N : x;
-                if (item.type == RT_BLUE_3 && !rwait(c3 + 1, B.n1, a.err)) break;
+                if (item.type == RT_BLUE_3 && !rwait(c3 + 1, B.n1 + a.wait_extra, a.err, a.wait_ticks)) break;
                 const int type = item.type, c0 = item.c0, nreal = L.n;
                 const cpx<T>* chirp = a.tw + L.choff;
                 rlines<T, true, true>(d, a.tplx, L1, a.tw, sm, np, item.cnt,
@@ -396,7 +400,7 @@ __global__ void __launch_bounds__(NTMAX) ragged_kernel(RArgs<T> a, int np) {
                 const int d = item.type == RT_BLUE_2 ? -1 : (item.type == RT_BLUE_4 ? 1 : dir);
                 cpx<T>* wsA = a.ws + a.wsoff[tr];
                 int* wc = item.type == RT_BLUE_4 ? c3 + 2 : c3 + 0;
-                if (!rwait(wc, n2, a.err)) break;
+                if (!rwait(wc, n2 + a.wait_extra, a.err, a.wait_ticks)) break;
                 const int type = item.type, c0 = item.c0, nreal = L.n;
                 const cpx<T>* chirp = a.tw + L.choff;
                 const cpx<T>* bspec = a.tw + L.bsoff;
@@ -789,6 +793,14 @@ RArgs<T> ragged_args(RaggedPlan<T>& P, cpx<T>* data, int dir, bool scaled) {
     a.dir = dir;
     a.scaled = scaled ? 1 : 0;
     a.tplx = P.tune.tplx;
+    // JT_RAGGED_FORCE_TIMEOUT=1 (tests of the failure path only): every
+    // dependency wait targets a count no producer reaches and gives up after
+    // 1 ms, so each consumer item is abandoned, the error flag is set and the
+    // kernel still exits normally
+    const char* ft = std::getenv("JT_RAGGED_FORCE_TIMEOUT");
+    const bool force = ft && ft[0] == '1';
+    a.wait_extra = force ? 1 : 0;
+    a.wait_ticks = force ? 100000ull : 200000000ull;
     a.ntr = P.ntr;
     return a;
 }

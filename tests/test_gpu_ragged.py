@@ -125,3 +125,26 @@ def test_ragged_autotune(orc, prec):
     q = jt.RaggedBatch1D(lens, precision=prec, autotune=True)
     assert "remembered" in q.describe()
     assert q.describe().split(" autotuned")[0] == d.split(" autotuned")[0]
+
+
+def test_ragged_wait_timeout_is_reported(orc, monkeypatch):
+    """Failure path of the bounded dependency waits, forced from the host:
+    every consumer item waits for a count no producer reaches, gives up after
+    1 ms, and the plan reports the failure; the next launch is clean."""
+    import torch
+    lens = np.array([12800, 65536, 12800], dtype=np.int64)   # four-step items
+    p = jt.RaggedBatch1D(lens, precision="f64")
+    x = orc.jrandom_doubles(20110603, int(2 * lens.sum()))
+    t = torch.from_numpy(x.copy()).cuda()
+    monkeypatch.setenv("JT_RAGGED_FORCE_TIMEOUT", "1")
+    p.complexForward(t)
+    with pytest.raises(jt.JTransformsError, match="timed out"):
+        p.sync_status()
+    monkeypatch.delenv("JT_RAGGED_FORCE_TIMEOUT")
+    t.copy_(torch.from_numpy(x))
+    p.complexForward(t)
+    p.sync_status()
+    y = t.cpu().numpy()
+    for i, n in enumerate(lens):
+        seg = slice(int(p.offsets[i]), int(p.offsets[i + 1]))
+        assert rel(y[seg], orc.complex_forward(int(n), x[seg])) <= tol(n, "f64")
