@@ -154,3 +154,28 @@ def test_r2r(orc, kind, dims, prec):
         p.forward(a, True)
     p.inverse(a, True)
     assert rel(a, x) <= 2 * tol(tot, prec)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("dims", [(6, 10), (100, 36), (5, 6, 7), (3, 12, 9), (16, 20)])
+def test_real_full_any_size(orc, dims, prec):
+    """realForwardFull / realInverseFull on 2D/3D sizes that are not powers of
+    two (the reference's mixed-radix full branch, DoubleFFT_2D.java:956-975):
+    the full complex spectrum of the real input, and back."""
+    tot = int(np.prod(dims))
+    x = orc.jrandom_doubles(SEED + 7, tot, signed=False).astype(DT[prec])
+    p = (FFT2 if len(dims) == 2 else FFT3)[prec](*dims)
+    full = np.zeros(2 * tot, DT[prec])
+    full[:tot] = x
+    p.realForwardFull(full)
+    src = np.concatenate([x, np.zeros_like(x)]).astype(np.float64)
+    ref = orc.fftnd_real_full(list(dims), src, -1)
+    e = rel(full, ref)
+    print(f"{prec} {dims} realForwardFull rel={e:.3e}", flush=True)
+    assert e <= tol(tot, prec)
+    back = np.zeros(2 * tot, DT[prec])
+    back[:tot] = x
+    p.realInverseFull(back, True)
+    e2 = rel(back, orc.fftnd_real_full(list(dims), src, 1, True))
+    print(f"{prec} {dims} realInverseFull(scale) rel={e2:.3e}", flush=True)
+    assert e2 <= tol(tot, prec)
