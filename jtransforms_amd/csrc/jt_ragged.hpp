@@ -47,6 +47,9 @@ struct RLen {
     int tpl;                    // fewest threads per line for the in-place stages
 };
 
+// Loads in flight per thread when a round's lines are read into LDS.
+constexpr int RLB = 8;
+
 // In-place stages hold at most 16 points per thread.
 template <typename T>
 constexpr int ragged_pts() { return 16; }
@@ -210,14 +213,14 @@ __device__ __forceinline__ void rlines(int dir, int tplx, const RLen& L, const c
         __syncthreads();
         if constexpr (LOAD_LF) {
             // lanes run across the G lines (adjacent columns in memory);
-            // four loads in flight per thread before the LDS writes
+            // RLB loads in flight per thread before the LDS writes
             const int nl = min(G, cnt - base);
             const int tot = nl * n;
             const float rnl = 1.0f / (float)nl;
-            for (int i0 = t; i0 < tot; i0 += 4 * NT) {
-                cpx<T> v[4];
+            for (int i0 = t; i0 < tot; i0 += RLB * NT) {
+                cpx<T> v[RLB];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < RLB; u++) {
                     const int idx = i0 + u * NT;
                     if (idx < tot) {
                         const int k = fdiv(idx, nl, rnl);
@@ -225,7 +228,7 @@ __device__ __forceinline__ void rlines(int dir, int tplx, const RLen& L, const c
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < RLB; u++) {
                     const int idx = i0 + u * NT;
                     if (idx < tot) {
                         const int k = fdiv(idx, nl, rnl), lg = idx - k * nl;
@@ -235,15 +238,15 @@ __device__ __forceinline__ void rlines(int dir, int tplx, const RLen& L, const c
                 }
             }
         } else if (act) {
-            for (int k0 = tl; k0 < n; k0 += 4 * tpl) {
-                cpx<T> v[4];
+            for (int k0 = tl; k0 < n; k0 += RLB * tpl) {
+                cpx<T> v[RLB];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < RLB; u++) {
                     const int k = k0 + u * tpl;
                     if (k < n) v[u] = load(line, k);
                 }
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < RLB; u++) {
                     const int k = k0 + u * tpl;
                     if (k < n) {
                         are[gpos(k)] = v[u].x;
