@@ -263,7 +263,7 @@ struct Fft1D {
     bool single() const { return kind == K_TPL || kind == K_GEN; }
     // workspace (complex elements) for `lines` contiguous lines
     int64_t workspace(int64_t lines) const {
-        if (kind == K_FOUR) return lines * n;
+        if (kind == K_FOUR) return lines * n + (p2->single() ? 0 : p2->workspace(lines * n1));
         if (kind == K_BLUE) return lines * m + pm->workspace(lines);
         return 0;
     }
@@ -299,6 +299,16 @@ std::unique_ptr<Fft1D<T>> make_fft1d(int64_t n) {
             if (!factor_radices(a, ra) || !factor_radices(b, rb)) continue;
             // prefer balanced split, n1 a power of two when possible
             if (!best1 || std::llabs(a - b) < std::llabs(best1 - n / best1)) best1 = a;
+        }
+        if (!best1) {
+            // n beyond gen_max^2: the largest single-kernel n1 whose cofactor
+            // is smooth; the n2-point row FFTs are then a plan of their own
+            // (four-step again), run in place on the workspace
+            for (int64_t a = gen_max<T>(); a >= 2 && !best1; a--) {
+                if (n % a) continue;
+                std::vector<int> ra, rb;
+                if (factor_radices(a, ra) && factor_radices(n / a, rb)) best1 = a;
+            }
         }
         if (best1) {
             p->kind = K_FOUR;
@@ -576,6 +586,21 @@ __global__ void blue_postmul(const cpx<T>* __restrict__ y, cpx<T>* __restrict__ 
     }
 }
 
+// out[b*n + k1 + n1*k2] = ws[b*n + k1*n2 + k2] * s (the last step of a
+// four-step transform whose n2-point rows ran as a plan of their own)
+template <typename T>
+__global__ void four_transpose(const cpx<T>* __restrict__ ws, cpx<T>* __restrict__ out, int64_t n1, int64_t n2,
+                               int64_t lines, T s) {
+    const int64_t n = n1 * n2;
+    const int64_t tot = lines * n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = i / n, r = i - b * n;
+        const int64_t k1 = r / n2, k2 = r - k1 * n2;
+        const cpx<T> v = ws[i];
+        out[b * n + k1 + n1 * k2] = cpx<T>{v.x * s, v.y * s};
+    }
+}
+
 template <typename T>
 __global__ void scale_kernel(cpx<T>* __restrict__ x, int64_t count, T s) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -632,13 +657,22 @@ void c2c_contig(const Fft1D<T>& p, const cpx<T>* in, cpx<T>* out, int64_t lines,
         return;
     }
     if (p.kind == K_FOUR) {
-        if (!p.p1->single() || !p.p2->single())
-            throw JtError(ST_UNSUPPORTED, "four-step sub-plan must be single-kernel");
+        if (!p.p1->single()) throw JtError(ST_UNSUPPORTED, "four-step column plan must be single-kernel");
         IOFourA<T, DIR> a{in, ws, p.hi.template as<cpx<T>>(), p.lo.template as<cpx<T>>(), lines * p.n2,
                           p.n, p.n1, p.n2, true};
         run_single<T, DIR, true>(*p.p1, a, st);
-        IOFourB<T> b{ws, out, lines * p.n1, p.n, p.n1, p.n2, scale, false};
-        run_single<T, DIR, false>(*p.p2, b, st);
+        if (p.p2->single()) {
+            IOFourB<T> b{ws, out, lines * p.n1, p.n, p.n1, p.n2, scale, false};
+            run_single<T, DIR, false>(*p.p2, b, st);
+            return;
+        }
+        // long rows: FFT them in place in the workspace, then transpose
+        c2c_contig<T, DIR>(*p.p2, ws, ws, lines * p.n1, T(1), ws + lines * p.n, st);
+        { JT_PROF(st, "four_transpose");
+        hipLaunchKernelGGL((four_transpose<T>), ew_grid(lines * p.n), dim3(256), 0, st, ws, out, p.n1, p.n2,
+                           lines, scale);
+        }
+        JT_HIP(hipGetLastError());
         return;
     }
     // Bluestein

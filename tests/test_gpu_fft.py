@@ -141,3 +141,40 @@ def test_batch_plan(orc, n, batch, prec):
     for b in (0, batch - 1):
         seg = slice(2 * n * b, 2 * n * (b + 1))
         assert rel(a[seg], orc.complex_forward(n, x[seg].astype(np.float64))) <= tol(n, prec)
+
+
+@pytest.mark.parametrize("n", [1 << 25, 3 << 22, 1000003])
+def test_large_1d_f64_vs_oracle(orc, n):
+    """Lengths beyond one four-step level (2^25 = 4096 x 8192, the 8192-point
+    rows a four-step plan of their own), a large balanced four-step, and a
+    large Bluestein length, against the oracle on the reference test's inputs."""
+    import torch
+    x = orc.jrandom_doubles(SEED, 2 * n)
+    t = torch.from_numpy(x).cuda()
+    jt.DoubleFFT_1D(n).complexForward(t)
+    torch.cuda.synchronize()
+    e = rel(t.cpu().numpy(), orc.complex_forward(n, x))
+    print(f"f64 n={n} complexForward rel={e:.3e}", flush=True)
+    assert e <= tol(n, "f64")
+
+
+def test_large_1d_f32_properties():
+    """FloatFFT_1D(2^27) (two four-step levels at FP32's 8192-point limit):
+    Parseval and the scaled round trip, properties that hold at any size."""
+    import torch
+    n = 1 << 27
+    g = torch.Generator(device="cuda")
+    g.manual_seed(SEED)
+    x = torch.randn(2 * n, dtype=torch.float32, device="cuda", generator=g)
+    a = x.clone()
+    p = jt.FloatFFT_1D(n)
+    p.complexForward(a)
+    ex = float(torch.sum(x.double() ** 2))
+    ea = float(torch.sum(a.double() ** 2))
+    par = abs(ea / (n * ex) - 1.0)
+    print(f"f32 n=2^27 Parseval deviation {par:.3e}", flush=True)
+    assert par <= 1e-5
+    p.complexInverse(a, True)
+    e = float(torch.linalg.vector_norm((a - x).double()) / torch.linalg.vector_norm(x.double()))
+    print(f"f32 n=2^27 round trip rel={e:.3e}", flush=True)
+    assert e <= 2 * tol(n, "f32")
