@@ -92,9 +92,9 @@ int64_t ws_need(jt_plan* p) {
     const int64_t tot = total(p) * p->batch;
     for (int a = 0; a < p->ndim; a++) {
         if (!I.ax[a]) continue;
+        // (real-input multi-D passes run the column axes on half-width data,
+        // which needs no more than the complex passes)
         int64_t lines = tot / p->dims[a];
-        w = std::max(w, axis_ws(*I.ax[a], lines));
-        // real-input multi-D passes run the column axes on half-width data
         w = std::max(w, axis_ws(*I.ax[a], lines));
     }
     if (I.real) {
