@@ -432,6 +432,13 @@ void go_tpl(IO io, const cpx<T>* tw, hipStream_t st) {
             return;
         }
     }
+    if constexpr (!COL && REAL && TL <= 16) {
+        // short rows of reals (DCT/DST/DHT x axis): one wave per workgroup
+        // (512^3 FP64 DCT row pass, 16 -> 8 -> 4 -> 2 lines per workgroup:
+        // 469, 448, 439, 617 us)
+        launch_tpl<T, DIR, N, TL, 64 / TL, COL, SPLIT, IO, Rs...>(io, tw, st);
+        return;
+    }
     launch_tpl<T, DIR, N, TL, S::W, COL, SPLIT, IO, Rs...>(io, tw, st);
 }
 
