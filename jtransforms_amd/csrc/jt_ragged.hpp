@@ -9,8 +9,9 @@
 //   SINGLE     a length that fits one workgroup's LDS
 //   FOUR_A/B   the column / row halves of a four-step length (n > LDS limit)
 //   BLUE_S     Bluestein with its convolution length m in LDS (all in one item)
-//   BLUE_1..4  Bluestein with a four-step m: premul+columns, rows+spectrum
-//              product, inverse columns, inverse rows+postmul
+//   BLUE_1..3  Bluestein with a four-step m: premul+columns; rows (forward
+//              FFT, spectrum product, inverse FFT, twiddle in one residency);
+//              inverse columns+postmul
 // Workgroups take items in queue order with an atomic counter.  An item that
 // consumes another phase's output waits on a per-transform counter; every
 // producer of that phase sits earlier in the queue and was therefore taken by
@@ -31,7 +32,7 @@ namespace jt {
 
 enum RKind { RK_SINGLE = 0, RK_FOUR = 1, RK_BLUE_S = 2, RK_BLUE_L = 3 };
 enum RType { RT_SINGLE = 0, RT_FOUR_A = 1, RT_FOUR_B = 2, RT_BLUE_S = 3, RT_BLUE_1 = 4, RT_BLUE_2 = 5,
-             RT_BLUE_3 = 6, RT_BLUE_4 = 7 };
+             RT_BLUE_3 = 6 };
 
 struct RLen {
     int n;
@@ -77,7 +78,7 @@ struct RArgs {
     int* err;      // set to 1 if a wait timed out
     int dir;       // -1 forward, +1 backward
     int scaled;    // multiply each transform by 1/n (scaled inverse)
-    int tplx;      // threads per line = (power of two >= n/16) << tplx
+    int tplx;      // threads per line = (the length's minimum, RLen::tpl) << tplx
     int wait_extra;          // 0; 1 makes every dependency target unreachable (failure-path test)
     uint64_t wait_ticks;     // bound of one wait, in 100 MHz wall-clock ticks
     int64_t ntr;
@@ -87,7 +88,7 @@ struct RArgs {
 //   wgs    persistent workgroups per CU, i.e. how the CU's LDS is split
 //          (each workgroup holds lines in its share)
 //   nt     tile shape: threads per workgroup (256, 512 or 1024)
-//   tplx   tile shape: threads per line, (power of two >= n/16) << tplx,
+//   tplx   tile shape: threads per line, (the length's minimum) << tplx,
 //          so lines per round = nt / threads per line (within the LDS share)
 //   chunk  lines per four-step work item, chunk * (single-kernel limit) points
 //   order  radix order of every runtime plan: 0 = 16, 8/4/2, then odd primes
@@ -190,9 +191,13 @@ __device__ __forceinline__ void stages_ip(int dir, const RLen& L, const cpx<T>* 
 // one line's slice of the two LDS arrays (re, im; each np_total reals).
 // load(line, k) / store(line, k, v) move point k of line `line` in [0, cnt).
 // All threads take every barrier.
-template <typename T, bool LOAD_LF, bool STORE_LF, class LoadF, class StoreF>
-__device__ __forceinline__ void rlines(int dir, int tplx, const RLen& L, const cpx<T>* __restrict__ tw, T* sm,
-                                       int np_total, int cnt, LoadF load, StoreF store) {
+// TWO: after the first transform (direction dir) mid(line, k, v) maps every
+// point in LDS and a second transform (direction dir2) follows before the
+// store (Bluestein rows: FFT, spectrum product, inverse FFT in one residency).
+template <typename T, bool LOAD_LF, bool STORE_LF, bool TWO, class LoadF, class MidF, class StoreF>
+__device__ __forceinline__ void rlines_impl(int dir, int dir2, int tplx, const RLen& L,
+                                            const cpx<T>* __restrict__ tw, T* sm, int np_total, int cnt,
+                                            LoadF load, MidF mid, StoreF store) {
     const int n = L.n;
     const int npl = n + (n >> 5) + 1;
     const int NT = blockDim.x;
@@ -256,6 +261,16 @@ __device__ __forceinline__ void rlines(int dir, int tplx, const RLen& L, const c
             }
         }
         stages_ip<T>(dir, L, tw, are, aim, tl, tpl, act);
+        if constexpr (TWO) {
+            if (act) {
+                for (int k = tl; k < n; k += tpl) {
+                    cpx<T> v = mid(line, k, cpx<T>{are[gpos(k)], aim[gpos(k)]});
+                    are[gpos(k)] = v.x;
+                    aim[gpos(k)] = v.y;
+                }
+            }
+            stages_ip<T>(dir2, L, tw, are, aim, tl, tpl, act);
+        }
         if constexpr (STORE_LF) {
             const int nl = min(G, cnt - base);
             const float rnl = 1.0f / (float)nl;
@@ -267,6 +282,13 @@ __device__ __forceinline__ void rlines(int dir, int tplx, const RLen& L, const c
             for (int k = tl; k < n; k += tpl) store(line, k, cpx<T>{are[gpos(k)], aim[gpos(k)]});
         }
     }
+}
+
+template <typename T, bool LOAD_LF, bool STORE_LF, class LoadF, class StoreF>
+__device__ __forceinline__ void rlines(int dir, int tplx, const RLen& L, const cpx<T>* __restrict__ tw, T* sm,
+                                       int np_total, int cnt, LoadF load, StoreF store) {
+    rlines_impl<T, LOAD_LF, STORE_LF, false>(dir, dir, tplx, L, tw, sm, np_total, cnt, load,
+                                             [](int, int, cpx<T> v) { return v; }, store);
 }
 
 template <typename T>
@@ -359,25 +381,24 @@ __global__ void __launch_bounds__(NTMAX) ragged_kernel(RArgs<T> a, int np) {
                 break;
             }
             case RT_FOUR_A:
-            case RT_BLUE_1:
-            case RT_BLUE_3: {
+            case RT_BLUE_1: {
                 // columns j2 in [c0, c0+cnt) of the n1 x n2 view; FFT_n1 then
                 // the exp(d 2 pi i j2 k1 / N) twiddle, into the workspace
-                const bool blue = item.type != RT_FOUR_A;
+                // (Bluestein: of the chirp-premultiplied, zero-padded input)
+                const bool blue = item.type == RT_BLUE_1;
                 const RLen& B = blue ? a.lens[L.lm] : L;  // the four-step length
                 const RLen& L1 = a.lens[B.l1];
                 const int n2 = B.n2;
                 const int64_t N = (int64_t)B.n1 * n2;
-                const int d = item.type == RT_BLUE_1 ? -1 : (item.type == RT_BLUE_3 ? 1 : dir);
+                const int d = blue ? -1 : dir;
                 cpx<T>* wsA = a.ws + a.wsoff[tr];
-                const cpx<T>* src = item.type == RT_BLUE_3 ? wsA + N : x;
-                if (item.type == RT_BLUE_3 && !rwait(c3 + 1, B.n1 + a.wait_extra, a.err, a.wait_ticks)) break;
-                const int type = item.type, c0 = item.c0, nreal = L.n;
+                const cpx<T>* src = x;
+                const int c0 = item.c0, nreal = L.n;
                 const cpx<T>* chirp = a.tw + L.choff;
                 rlines<T, true, true>(d, a.tplx, L1, a.tw, sm, np, item.cnt,
                           [=](int line, int j1) {
                               const int e = j1 * n2 + (c0 + line);
-                              if (type == RT_BLUE_1) {
+                              if (blue) {
                                   if (e >= nreal) return cpx<T>{T(0), T(0)};
                                   return dir < 0 ? cmulc(src[e], chirp[e]) : cmul(src[e], chirp[e]);
                               }
@@ -387,42 +408,72 @@ __global__ void __launch_bounds__(NTMAX) ragged_kernel(RArgs<T> a, int np) {
                               int j2 = c0 + line;
                               wsA[k1 * n2 + j2] = cmul(v, rtw2<T>(d, a.tw, B, j2 * k1, N));
                           });
-                rsignal(item.type == RT_BLUE_3 ? c3 + 2 : c3 + 0, item.cnt);
+                rsignal(c3 + 0, item.cnt);
                 break;
             }
-            case RT_FOUR_B:
-            case RT_BLUE_2:
-            case RT_BLUE_4: {
+            case RT_FOUR_B: {
                 // rows k1 in [c0, c0+cnt): FFT_n2 of the workspace row; output
                 // point k2 is element k1 + n1*k2
-                const bool blue = item.type != RT_FOUR_B;
-                const RLen& B = blue ? a.lens[L.lm] : L;
+                const RLen& L2 = a.lens[L.l2];
+                const int n1 = L.n1, n2 = L.n2;
+                cpx<T>* wsA = a.ws + a.wsoff[tr];
+                if (!rwait(c3 + 0, n2 + a.wait_extra, a.err, a.wait_ticks)) break;
+                const int c0 = item.c0;
+                rlines<T, false, true>(dir, a.tplx, L2, a.tw, sm, np, item.cnt,
+                          [=](int line, int j2) { return wsA[(c0 + line) * n2 + j2]; },
+                          [=](int line, int k2, cpx<T> v) {
+                              x[(c0 + line) + n1 * k2] = cpx<T>{v.x * scale, v.y * scale};
+                          });
+                break;
+            }
+            case RT_BLUE_2: {
+                // Bluestein rows k1 in [c0, c0+cnt) of the convolution length
+                // N = n1*n2, in one LDS residency: forward FFT_n2 (point k2 is
+                // X[k1 + n1*k2]), times the kernel spectrum, inverse FFT_n2 back
+                // to j2, times exp(+2 pi i j2 k1 / N); written back in place.
+                // (The inverse FFT_N then finishes with FFT_n1 over k1.)
+                const RLen& B = a.lens[L.lm];
                 const RLen& L2 = a.lens[B.l2];
                 const int n1 = B.n1, n2 = B.n2;
                 const int64_t N = (int64_t)n1 * n2;
-                const int d = item.type == RT_BLUE_2 ? -1 : (item.type == RT_BLUE_4 ? 1 : dir);
                 cpx<T>* wsA = a.ws + a.wsoff[tr];
-                int* wc = item.type == RT_BLUE_4 ? c3 + 2 : c3 + 0;
-                if (!rwait(wc, n2 + a.wait_extra, a.err, a.wait_ticks)) break;
-                const int type = item.type, c0 = item.c0, nreal = L.n;
-                const cpx<T>* chirp = a.tw + L.choff;
+                if (!rwait(c3 + 0, n2 + a.wait_extra, a.err, a.wait_ticks)) break;
+                const int c0 = item.c0;
                 const cpx<T>* bspec = a.tw + L.bsoff;
-                rlines<T, false, true>(d, a.tplx, L2, a.tw, sm, np, item.cnt,
+                rlines_impl<T, false, false, true>(-1, 1, a.tplx, L2, a.tw, sm, np, item.cnt,
                           [=](int line, int j2) { return wsA[(c0 + line) * n2 + j2]; },
                           [=](int line, int k2, cpx<T> v) {
-                              const int K = (c0 + line) + n1 * k2;
-                              if (type == RT_FOUR_B) {
-                                  x[K] = cpx<T>{v.x * scale, v.y * scale};
-                              } else if (type == RT_BLUE_2) {
-                                  cpx<T> bb = bspec[K];
-                                  wsA[N + K] = dir < 0 ? cmul(v, bb) : cmulc(v, bb);
-                              } else if (K < nreal) {
-                                  cpx<T> c = chirp[K];
+                              const cpx<T> bb = bspec[(c0 + line) + n1 * k2];
+                              return dir < 0 ? cmul(v, bb) : cmulc(v, bb);
+                          },
+                          [=](int line, int j2, cpx<T> v) {
+                              const int k1 = c0 + line;
+                              wsA[k1 * n2 + j2] = cmul(v, rtw2<T>(1, a.tw, B, j2 * k1, N));
+                          });
+                rsignal(c3 + 1, item.cnt);
+                break;
+            }
+            case RT_BLUE_3: {
+                // Bluestein columns j2 in [c0, c0+cnt): inverse FFT_n1 over k1
+                // gives point j1*n2 + j2 of the convolution; chirp postmul of
+                // the first n points into the transform
+                const RLen& B = a.lens[L.lm];
+                const RLen& L1 = a.lens[B.l1];
+                const int n1 = B.n1, n2 = B.n2;
+                cpx<T>* wsA = a.ws + a.wsoff[tr];
+                if (!rwait(c3 + 1, n1 + a.wait_extra, a.err, a.wait_ticks)) break;
+                const int c0 = item.c0, nreal = L.n;
+                const cpx<T>* chirp = a.tw + L.choff;
+                rlines<T, true, true>(1, a.tplx, L1, a.tw, sm, np, item.cnt,
+                          [=](int line, int k1) { return wsA[k1 * n2 + (c0 + line)]; },
+                          [=](int line, int j1, cpx<T> v) {
+                              const int K = j1 * n2 + (c0 + line);
+                              if (K < nreal) {
+                                  const cpx<T> c = chirp[K];
                                   v = dir < 0 ? cmulc(v, c) : cmul(v, c);
                                   x[K] = cpx<T>{v.x * scale, v.y * scale};
                               }
                           });
-                if (item.type == RT_BLUE_2) rsignal(c3 + 1, item.cnt);
                 break;
             }
             case RT_BLUE_S: {
@@ -676,14 +727,14 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
             wtot += L.n;
         } else if (L.kind == RK_BLUE_L) {
             wsoff[i] = wtot;
-            wtot += 2 * (int64_t)L.m;
+            wtot += (int64_t)L.m;
         }
     }
     P->total = tot;
     P->lens_of.assign(lengths, lengths + count);
 
     // work items: cost-balanced phases interleaved with single items
-    std::vector<RItem> ph[4], singles;
+    std::vector<RItem> ph[3], singles;
     // four-step items carry ~4 LDS loads of lines each
     const int CHUNK = (int)(tune.chunk * lim);
     auto split_items = [&](std::vector<RItem>& v, int type, int len, int64_t tr, int total_lines, int line_len) {
@@ -707,7 +758,6 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
             split_items(ph[0], RT_BLUE_1, tlen[i], i, M.n2, M.n1);
             split_items(ph[1], RT_BLUE_2, tlen[i], i, M.n1, M.n2);
             split_items(ph[2], RT_BLUE_3, tlen[i], i, M.n2, M.n1);
-            split_items(ph[3], RT_BLUE_4, tlen[i], i, M.n1, M.n2);
         }
     }
     for (auto& kv : by_len) {
@@ -723,8 +773,8 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
     }
     std::stable_sort(sgl.begin(), sgl.end(), [](auto& a, auto& b) { return a.first > b.first; });
     std::vector<RItem> items;
-    const size_t q = (sgl.size() + 3) / 4;
-    for (int p = 0; p < 4; p++) {
+    const size_t q = (sgl.size() + 2) / 3;
+    for (int p = 0; p < 3; p++) {
         items.insert(items.end(), ph[p].begin(), ph[p].end());
         for (size_t s = p * q; s < std::min(sgl.size(), (p + 1) * q); s++) items.push_back(sgl[s].second);
     }
