@@ -1,6 +1,7 @@
 // C ABI of the transform library (declared in include/jtransforms_amd.h).
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -233,10 +234,27 @@ int jt_execute(jt_plan* p, int op, void* data, int64_t offset, int scale, void* 
     return guarded([&]() -> int {
         if (!p || !data) return set_err(JT_BAD_ARG, "null argument");
         if (offset < 0) return set_err(JT_BAD_ARG, "negative offset");
-        if (jt_plan_elements(p, op) < 0) return set_err(JT_BAD_ARG, "op not defined for this plan");
+        const int64_t ne = jt_plan_elements(p, op);
+        if (ne < 0) return set_err(JT_BAD_ARG, "op not defined for this plan");
         hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-        if (p->precision == JT_F64) return exec_t<double>(p, op, static_cast<double*>(data) + offset, scale != 0, st);
-        return exec_t<float>(p, op, static_cast<float*>(data) + offset, scale != 0, st);
+        const size_t es = p->precision == JT_F64 ? 8 : 4;
+        char* ptr = static_cast<char*>(data) + offset * (int64_t)es;
+        auto run = [&](void* d) -> int {
+            if (p->precision == JT_F64) return exec_t<double>(p, op, static_cast<double*>(d), scale != 0, st);
+            return exec_t<float>(p, op, static_cast<float*>(d), scale != 0, st);
+        };
+        if (reinterpret_cast<uintptr_t>(ptr) % (2 * es) == 0) return run(ptr);
+        // The kernels access (re, im) pairs of 2*es bytes: an odd offset (the
+        // reference accepts any offa) runs on an aligned copy of the range,
+        // staged on the same stream.
+        DevBuf& stg = p->precision == JT_F64 ? p->d->staging : p->f->staging;
+        const size_t bytes = (size_t)ne * es;
+        if (stg.bytes < bytes) stg.alloc(bytes);
+        JT_HIP(hipMemcpyAsync(stg.p, ptr, bytes, hipMemcpyDeviceToDevice, st));
+        const int rc = run(stg.p);
+        if (rc != JT_OK) return rc;
+        JT_HIP(hipMemcpyAsync(ptr, stg.p, bytes, hipMemcpyDeviceToDevice, st));
+        return JT_OK;
     });
 }
 
