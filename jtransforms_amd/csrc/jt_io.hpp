@@ -41,6 +41,7 @@ enum { NT_LOAD = 1, NT_STORE = 2 };
 template <typename T, int NT = 0>
 struct IOStrided {
     static constexpr bool pre_lds = false, post_lds = false;
+    static constexpr int streaming = NT;
     static const char* tag() { return NT == 0 ? "c2c" : "c2c,nt"; }
     std::string desc() const { return ",es=" + std::to_string(m.es); }
     const cpx<T>* in;

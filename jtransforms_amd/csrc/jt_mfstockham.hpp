@@ -95,7 +95,10 @@ __global__ void __launch_bounds__(N / 16 * W) mfma_stockham_kernel(IO io, const 
         for (int q = 0; q < 4; q++) {
             const float* src = reinterpret_cast<const float*>(io.in + tl_base[q]) + 2 * (tl_j[q] + rh * TL) + part;
 #pragma unroll
-            for (int u = 0; u < 8; u++) b[q][u] = src[2 * TL * u];
+            for (int u = 0; u < 8; u++) {
+                if constexpr ((IO::streaming & NT_LOAD) != 0) b[q][u] = __builtin_nontemporal_load(src + 2 * TL * u);
+                else b[q][u] = src[2 * TL * u];
+            }
         }
     } else {
         // coalesced across adjacent lines, staged through LDS
@@ -132,7 +135,11 @@ __global__ void __launch_bounds__(N / 16 * W) mfma_stockham_kernel(IO io, const 
                 if (tl_act[q]) {
 #pragma unroll
                     for (int u = 0; u < 4; u++)
-                        dst[TL * u] = cpx<float>{outr[q][u] * io.scale, outi[q][u] * io.scale};
+                    {
+                        const cpx<float> o{outr[q][u] * io.scale, outi[q][u] * io.scale};
+                        if constexpr ((IO::streaming & NT_STORE) != 0) st_nt(dst + TL * u, o);
+                        else dst[TL * u] = o;
+                    }
                 }
             }
         } else {
