@@ -106,6 +106,7 @@ void run_real_forward_nd(jt_plan* p, T* a, hipStream_t st) {
     if (p->ndim == 2) {
         const int64_t r = p->dims[0], c = p->dims[1], ch = c / 2;
         real_forward<T>(*I.real, a, r, ws, st);
+        if (col_fft_fix2d<T, -1, 1>(*I.ax[0], z, r, ch, T(1), st)) return;
         axis_c2c<T, -1>(*I.ax[0], z, LineMap{ch, ch, 1, 0, ch}, T(1), ws, st);
         { JT_PROF(st, "rdft2d_fix");
         hipLaunchKernelGGL((rdft2d_fix<T>), ew_grid(r / 2), dim3(256), 0, st, a, r, c, 1);
@@ -130,11 +131,14 @@ void run_real_inverse_nd(jt_plan* p, T* a, bool scale, hipStream_t st) {
     cpx<T>* z = reinterpret_cast<cpx<T>*>(a);
     if (p->ndim == 2) {
         const int64_t r = p->dims[0], c = p->dims[1], ch = c / 2;
-        { JT_PROF(st, "rdft2d_fix");
-        hipLaunchKernelGGL((rdft2d_fix<T>), ew_grid(r / 2), dim3(256), 0, st, a, r, c, -1);
+        const T cs = scale ? T(1.0 / (double)r) : T(1);
+        if (!col_fft_fix2d<T, 1, -1>(*I.ax[0], z, r, ch, cs, st)) {
+            { JT_PROF(st, "rdft2d_fix");
+            hipLaunchKernelGGL((rdft2d_fix<T>), ew_grid(r / 2), dim3(256), 0, st, a, r, c, -1);
+            }
+            JT_HIP(hipGetLastError());
+            axis_c2c<T, 1>(*I.ax[0], z, LineMap{ch, ch, 1, 0, ch}, cs, ws, st);
         }
-        JT_HIP(hipGetLastError());
-        axis_c2c<T, 1>(*I.ax[0], z, LineMap{ch, ch, 1, 0, ch}, scale ? T(1.0 / (double)r) : T(1), ws, st);
         real_inverse<T>(*I.real, a, r, scale, ws, st);
         return;
     }

@@ -70,6 +70,37 @@ struct IOStrided {
     __device__ void post(int64_t, int64_t, const T*, const T*, int, int) const {}
 };
 
+// The column pass of a packed-real 2D transform with the rdft2d_sub fix of
+// column 0 (reference: DoubleFFT_2D.java:2544-2574) done by the workgroup
+// that transforms that column.  Column 0 holds the transform of
+// Y[.][0] + i Y[.][cols/2]; for rows 0 < i < rows/2 and j = rows - i:
+//   ISGN > 0 (realForward, after the FFT):  Z_j <- (Z_i - conj Z_j) / 2,
+//                                           Z_i <- Z_i - Z_j(new);
+//   ISGN < 0 (realInverse, before the FFT): Z_i <- Z_i + Z_j,
+//                                           Z_j <- (Re(Z_i - Z_j), Im(Z_j - Z_i)).
+template <typename T, int ISGN>
+struct IOStridedFix2D : IOStrided<T> {
+    static constexpr int line0_hook = ISGN;
+    static const char* tag() { return ISGN > 0 ? "c2c,rdft2d" : "c2c,rdft2d_inv"; }
+    int64_t rows;
+    __device__ void line0(int tid, int nthreads) const {
+        cpx<T>* col = this->out + this->m.base(0);
+        const int64_t es = this->m.es;
+        for (int64_t i = 1 + tid; i < rows / 2; i += nthreads) {
+            const int64_t j = rows - i;
+            const cpx<T> zi = col[i * es], zj = col[j * es];
+            if constexpr (ISGN > 0) {
+                const cpx<T> nj{T(0.5) * (zi.x - zj.x), T(0.5) * (zi.y + zj.y)};
+                col[j * es] = nj;
+                col[i * es] = cpx<T>{zi.x - nj.x, zi.y - nj.y};
+            } else {
+                col[i * es] = cpx<T>{zi.x + zj.x, zi.y + zj.y};
+                col[j * es] = cpx<T>{zi.x - zj.x, zj.y - zi.y};
+            }
+        }
+    }
+};
+
 // exp(DIR 2 pi i e / n) from two tables: hi[e >> 8] * lo[e & 255], both
 // holding forward values exp(-2 pi i x/n).
 template <int DIR, typename T>

@@ -62,6 +62,18 @@ int64_t axis_ws(const Fft1D<T>& p, int64_t lines) {
     return lines * p.n + p.workspace(lines);
 }
 
+// Column pass of a 2D realForward (ISGN > 0, DIR -1) or realInverse (ISGN < 0,
+// DIR +1) with the rdft2d_sub fix of column 0 done by the workgroup that
+// transforms it (one launch instead of two).  Only for a column plan that is
+// one compile-time Stockham kernel; returns false, having launched nothing,
+// for any other plan.
+template <typename T, int DIR, int ISGN>
+bool col_fft_fix2d(const Fft1D<T>& p, cpx<T>* z, int64_t rows, int64_t ch, T scale, hipStream_t st) {
+    if (p.kind != K_TPL || p.mf || p.n != rows || ch <= 0) return false;
+    IOStridedFix2D<T, ISGN> io{{z, z, LineMap{ch, ch, 1, 0, ch}, ch, scale, true}, rows};
+    return dispatch_tpl_n<T, DIR, true>(rows, io, p.tw.template as<cpx<T>>(), st);
+}
+
 // rdft2d_sub (DoubleFFT_2D.java:2544-2574) on the packed column pair 0/1:
 // isgn > 0 splits the column-0 transform of (Y[.][0] + i Y[.][c/2]) into the
 // documented layout; isgn < 0 recombines it.

@@ -16,6 +16,8 @@
 // one padded LDS buffer; with SPLIT the real and imaginary halves are
 // exchanged in two rounds so the buffer is half the size (more waves per CU).
 #pragma once
+#include <type_traits>
+
 #include "jt_dft.hpp"
 
 namespace jt {
@@ -203,6 +205,15 @@ __device__ __forceinline__ int64_t xcd_group(int64_t bid, int64_t ngroups) {
     return (bid % nx) * per + bid / nx;
 }
 
+// IO policies that finish line 0 in place declare `static constexpr int
+// line0_hook` (> 0: after the line's stores, < 0: before its loads) and
+// `__device__ void line0(int tid, int nthreads)`.
+template <class IO, class = void>
+struct io_line0_hook : std::integral_constant<int, 0> {};
+template <class IO>
+struct io_line0_hook<IO, std::void_t<decltype(IO::line0_hook)>>
+    : std::integral_constant<int, IO::line0_hook> {};
+
 // ----------------------------------------------------------------------------
 // The kernel.  IO supplies:
 //   int64_t nlines;  bool xcd;
@@ -238,6 +249,15 @@ __global__ void __launch_bounds__(TL* W) stockham_kernel(IO io, const cpx<T>* __
     const bool active = line < io.nlines;
     const int64_t base = active ? io.base(line) : 0;
 
+    if constexpr (io_line0_hook<IO>::value < 0) {
+        // the workgroup that holds line 0 rewrites it before anyone loads it
+        if (g == 0) {
+            io.line0(tid, TL * W);
+            __syncthreads();
+            __atomic_thread_fence(__ATOMIC_ACQ_REL);
+        }
+    }
+
     cpx<T> v[E];
     if constexpr (IO::pre_lds) {
         if (active) io.template pre<N, TL>(line, base, lds_re + w * G::pitch, lds_im + w * G::pitch, t, TL);
@@ -269,6 +289,16 @@ __global__ void __launch_bounds__(TL* W) stockham_kernel(IO io, const cpx<T>* __
                     int k = t + TL * (q + r * (E / RL));
                     io.store(base, k, v[q * RL + r]);
                 }
+        }
+    }
+    if constexpr (io_line0_hook<IO>::value > 0) {
+        // the workgroup that holds line 0 finishes it once all its stores
+        // are done; the fence orders them before the re-reads and drops L1
+        // lines loaded at the start
+        if (g == 0) {
+            __syncthreads();
+            __atomic_thread_fence(__ATOMIC_ACQ_REL);
+            io.line0(tid, TL * W);
         }
     }
 }

@@ -85,3 +85,17 @@ def test_split_fp16_fft3d(orc, dims):
     e = rel(a, ref)
     print(f"FloatFFT_3D{dims} split-FP16 rel={e:.3e}", flush=True)
     assert e <= SPLIT_TOL
+
+
+def test_split_fp16_fft2d_real(orc):
+    # the column pass runs on the matrix-core kernel; the column-0 split
+    # then runs as its own launch
+    rows, cols = 1024, 64
+    x = std_normal(rows * cols, 20110626)
+    p = jt.FloatFFT_2D(rows, cols)
+    p.set_split_fp16(True)
+    a = x.copy()
+    p.realForward(a)
+    e = rel(a, orc.fft2d_real_forward(rows, cols, x.astype(np.float64)))
+    print(f"FloatFFT_2D({rows},{cols}) realForward split-FP16 rel={e:.3e}", flush=True)
+    assert e <= SPLIT_TOL

@@ -47,7 +47,10 @@ def test_fft2d_complex(orc, rows, cols, prec):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("rows,cols", [(2, 2), (4, 8), (16, 4), (64, 64), (256, 1024), (4096, 64)])
+# every compile-time column length (the column-0 split runs inside the column
+# kernel) and lengths on the other plans (2, 4: generic; 8192: four-step)
+@pytest.mark.parametrize("rows,cols", [(2, 2), (4, 8), (16, 4), (32, 8), (64, 64), (128, 16), (256, 1024),
+                                       (512, 8), (1024, 4), (2048, 16), (4096, 64), (8192, 4)])
 def test_fft2d_real(orc, rows, cols, prec):
     x = orc.jrandom_doubles(SEED, rows * cols, signed=False).astype(DT[prec])
     ref = orc.fft2d_real_forward(rows, cols, x.astype(np.float64))
