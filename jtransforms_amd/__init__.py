@@ -22,7 +22,8 @@ __all__ = [
     "DoubleDCT_1D", "FloatDCT_1D", "DoubleDCT_2D", "FloatDCT_2D", "DoubleDCT_3D", "FloatDCT_3D",
     "DoubleDST_1D", "FloatDST_1D", "DoubleDST_2D", "FloatDST_2D", "DoubleDST_3D", "FloatDST_3D",
     "DoubleDHT_1D", "FloatDHT_1D", "DoubleDHT_2D", "FloatDHT_2D", "DoubleDHT_3D", "FloatDHT_3D",
-    "FFTBatch1D", "RaggedBatch1D", "IllegalArgumentException", "JTransformsError", "lib", "LIB_PATH",
+    "FFTBatch1D", "RaggedBatch1D", "RealFFTUtils_2D", "RealFFTUtils_3D", "IllegalArgumentException",
+    "JTransformsError", "lib", "LIB_PATH",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -377,3 +378,6 @@ def build():
     jobs = os.environ.get("MAX_JOBS", "8")
     subprocess.run(["make", "-s", "-j", jobs, "-C", os.path.join(_HERE, "csrc")], check=True)
     return LIB_PATH
+
+
+from .realfftutils import RealFFTUtils_2D, RealFFTUtils_3D  # noqa: E402  (host-side index helpers)

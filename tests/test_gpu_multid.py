@@ -182,3 +182,26 @@ def test_real_full_any_size(orc, dims, prec):
     e2 = rel(back, orc.fftnd_real_full(list(dims), src, 1, True))
     print(f"{prec} {dims} realInverseFull(scale) rel={e2:.3e}", flush=True)
     assert e2 <= tol(tot, prec)
+
+
+# RealFFTUtils_2DTest / RealFFTUtils_3DTest (fft/RealFFTUtils_2DTest.java:142-166): every mode unpacked from
+# the packed realForward output equals the realForwardFull output
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("dims", [(16, 32), (64, 8), (8, 16, 32), (32, 4, 16)])
+def test_realfftutils_unpack_matches_full(orc, dims, prec):
+    tot = int(np.prod(dims))
+    x = orc.jrandom_doubles(SEED + tot, tot).astype(DT[prec])
+    p = (FFT2 if len(dims) == 2 else FFT3)[prec](*dims)
+    u = (jt.RealFFTUtils_2D if len(dims) == 2 else jt.RealFFTUtils_3D)(*dims)
+    packed = x.copy()
+    p.realForward(packed)
+    full = np.zeros(2 * tot, DT[prec])
+    full[:tot] = x
+    p.realForwardFull(full)
+    full = full.reshape(dims[:-1] + (2 * dims[-1],))
+    got = np.zeros_like(full)
+    for idx in np.ndindex(*full.shape):
+        got[idx] = u.unpack(*idx, packed)
+    e = rel(got, full)
+    print(f"{prec} RealFFTUtils {dims} rel={e:.3e}", flush=True)
+    assert e <= tol(tot, prec)
