@@ -154,7 +154,11 @@ class _Plan:
         return buf.value.decode()
 
     def elements(self, op: int) -> int:
-        return int(lib().jt_plan_elements(self._h, op))
+        cache = self.__dict__.setdefault("_elements", {})
+        n = cache.get(op)
+        if n is None:
+            n = cache[op] = int(lib().jt_plan_elements(self._h, op))
+        return n
 
     def _run(self, op: int, a, offa: int = 0, scale: bool = False):
         L = lib()
@@ -170,10 +174,16 @@ class _Plan:
                 raise TypeError("tensor must be contiguous")
             if a.numel() - offa < need:
                 raise IndexError(f"array too short: need {need} elements from offset {offa}")
-            stream = torch.cuda.current_stream(a.device).cuda_stream
-            with torch.cuda.device(a.device):
+            if a.device.index == torch.cuda.current_device():
+                # the common case: no device switch around the call
+                stream = torch.cuda.current_stream().cuda_stream
                 rc = L.jt_execute(self._h, op, ctypes.c_void_p(a.data_ptr()), offa, int(bool(scale)),
                                   ctypes.c_void_p(stream))
+            else:
+                with torch.cuda.device(a.device):
+                    stream = torch.cuda.current_stream().cuda_stream
+                    rc = L.jt_execute(self._h, op, ctypes.c_void_p(a.data_ptr()), offa, int(bool(scale)),
+                                      ctypes.c_void_p(stream))
         else:
             if not isinstance(a, np.ndarray) or a.dtype != self.dtype or not a.flags.c_contiguous:
                 raise TypeError(f"expected a contiguous numpy {np.dtype(self.dtype).name} array")
