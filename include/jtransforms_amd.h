@@ -17,10 +17,15 @@
  *   - size errors return JT_BAD_SIZE / JT_NOT_POW2 where the reference throws
  *     IllegalArgumentException (the binding maps them to that exception with
  *     the text of jt_last_error()).
- * Plans are immutable after creation and may be used from several host
- * threads on different streams, except that one plan's workspace is shared:
- * calls on one plan must be ordered on one stream (or serialised by the
- * caller).
+ * Re-entrancy: one plan may be used from several host threads and streams at
+ * once, as the reference's 1D plans may (they allocate their scratch per
+ * call, fft/DoubleFFT_1D.java:1922,6639).  The plan owns one device buffer,
+ * sized at creation, holding each op's workspace and staging area; calls
+ * that use it are serialised (the host side under a per-plan lock, the
+ * device side by an event chain across streams), calls that do not (most
+ * single-pass transforms on device memory) run concurrently.  Under the
+ * caller's stream capture the captured graph orders the work instead.
+ * Execution never allocates device memory.
  */
 #ifndef JTRANSFORMS_AMD_H
 #define JTRANSFORMS_AMD_H

@@ -160,7 +160,67 @@ class _Plan:
             n = cache[op] = int(lib().jt_plan_elements(self._h, op))
         return n
 
+    # -- the array-of-arrays overloads (double[][] / double[][][]) ----------
+    def _nested_shape(self, op: int):
+        """Shape of the reference's double[][] / double[][][] argument for op,
+        or None where the class has no such overload."""
+        return None
+
+    @staticmethod
+    def _is_nested(a) -> bool:
+        if isinstance(a, (list, tuple)):
+            return True
+        if isinstance(a, np.ndarray):
+            return a.ndim > 1
+        return _is_torch(a) and a.dim() > 1
+
+    def _run_nested(self, op: int, a, scale: bool):
+        """Runs op on a 2D/3D array: a C-contiguous numpy array or device tensor
+        of the overload's shape (transformed in place through a flat view), or
+        nested lists of 1D rows, like Java's double[][] (gathered, transformed,
+        and written back into every row in place).  For realForwardFull /
+        realInverseFull each row holds 2*columns values of which the first
+        columns are the real input (DoubleFFT_2D.java:1010-1041,
+        DoubleFFT_3D.java realForwardFull(double[][][])); the flat overloads
+        take the rows*columns reals compactly instead."""
+        shape = self._nested_shape(op)
+        if shape is None:
+            raise TypeError(f"{type(self).__name__} has no array-of-arrays overload")
+        shape = tuple(int(d) for d in shape)
+        rows_of = None
+        if isinstance(a, (list, tuple)):
+            rows_of = a
+            work = np.array(a, dtype=self.dtype)
+        else:
+            work = a
+        if tuple(work.shape) != shape:
+            raise IndexError(f"expected an array of shape {shape}, got {tuple(work.shape)}")
+        if _is_torch(work):
+            if not work.is_contiguous():
+                raise TypeError("tensor must be contiguous")
+            flat = work.view(-1)
+        else:
+            if not work.flags.c_contiguous or work.dtype != self.dtype:
+                raise TypeError(f"expected a C-contiguous {np.dtype(self.dtype).name} array")
+            flat = work.reshape(-1)
+        if op in (REAL_FORWARD_FULL, REAL_INVERSE_FULL):
+            cols = shape[-1] // 2
+            buf = flat.clone() if _is_torch(flat) else flat.copy()
+            reals = work[..., :cols].reshape(-1)
+            buf[: reals.shape[0]] = reals
+            self._run(op, buf, 0, scale)
+            flat[:] = buf
+        else:
+            self._run(op, flat, 0, scale)
+        if rows_of is not None:
+            _write_back(rows_of, work)
+
     def _run(self, op: int, a, offa: int = 0, scale: bool = False):
+        if self._is_nested(a):
+            if offa:
+                raise TypeError("the array-of-arrays overloads take no offset")
+            self._run_nested(op, a, scale)
+            return
         L = lib()
         need = self.elements(op)
         if _is_torch(a):
@@ -193,6 +253,18 @@ class _Plan:
                                    int(bool(scale)))
         if rc:
             _raise(rc)
+
+
+def _write_back(rows_of, values):
+    """Copies values (an array of the nested shape) into the caller's nested
+    lists of rows, row by row and in place."""
+    for i, row in enumerate(rows_of):
+        if isinstance(row, (list, tuple)):
+            _write_back(row, values[i])
+        elif isinstance(row, np.ndarray):
+            row[...] = values[i]
+        else:
+            raise TypeError("rows of an array-of-arrays argument must be numpy arrays")
 
 
 def _offa_scale(args, name):
@@ -255,6 +327,12 @@ class DoubleFFT_2D(_FFTMethods):
         self.rows, self.columns = int(rows), int(columns)
         super().__init__(lib().jt_plan_create_fft2d, self.rows, self.columns)
 
+    def _nested_shape(self, op):
+        # double[][] overloads (DoubleFFT_2D.java:345 complexForward, :924
+        # realForward, ...): rows x 2*columns complex, rows x columns packed real
+        c = self.columns if op in (REAL_FORWARD, REAL_INVERSE) else 2 * self.columns
+        return (self.rows, c)
+
 
 class FloatFFT_2D(DoubleFFT_2D):
     _precision = JT_F32
@@ -267,6 +345,12 @@ class DoubleFFT_3D(_FFTMethods):
     def __init__(self, slices: int, rows: int, columns: int):
         self.slices, self.rows, self.columns = int(slices), int(rows), int(columns)
         super().__init__(lib().jt_plan_create_fft3d, self.slices, self.rows, self.columns)
+
+    def _nested_shape(self, op):
+        # double[][][] overloads (DoubleFFT_3D.java:542 complexForward, :1467
+        # realForward, ...)
+        c = self.columns if op in (REAL_FORWARD, REAL_INVERSE) else 2 * self.columns
+        return (self.slices, self.rows, c)
 
 
 class FloatFFT_3D(DoubleFFT_3D):
@@ -318,6 +402,11 @@ class _R2R(_Plan):
         self.dims = tuple(int(d) for d in dims)
         arr = (ctypes.c_int64 * len(dims))(*self.dims)
         super().__init__(getattr(lib(), self._creator), len(dims), arr)
+
+    def _nested_shape(self, op):
+        # double[][] / double[][][] overloads of the 2D/3D classes
+        # (DoubleDCT_2D.java:280 forward, :544 inverse; DoubleDHT_2D.java:286)
+        return self.dims if len(self.dims) > 1 else None
 
     def forward(self, a, *args):
         if self._has_scale_fwd:

@@ -67,8 +67,15 @@ int create(jt_plan** out, int family, int precision, int ndim, const int64_t* di
             if (dims[i] > (int64_t(1) << 40) / tot) return set_err(JT_TOO_BIG, "data array is too big");
             tot *= dims[i];
         }
-        if (family == FAM_DCT && ndim == 1 && dims[0] > (int64_t(1) << 28))
-            return set_err(JT_TOO_BIG, "n must be smaller or equal to 268435456");
+        // DoubleDCT_1D (dct/DoubleDCT_1D.java:93-100), and DoubleDST_1D through
+        // the DoubleDCT_1D it builds (dst/DoubleDST_1D.java:59-65), reject
+        // 2^28 < n unless they run on large arrays, which they do for lengths
+        // beyond a 32-bit Java array; this library has no array-size limit
+        // of its own and accepts those
+        if ((family == FAM_DCT || family == FAM_DST) && ndim == 1 && dims[0] > (int64_t(1) << 28) &&
+            dims[0] <= INT32_MAX)
+            return set_err(JT_TOO_BIG,
+                           "n must be smaller or equal to 268435456 when useLargeArrays argument is set to false");
         auto p = std::make_unique<jt_plan>();
         p->family = family;
         p->precision = precision;
@@ -82,9 +89,76 @@ int create(jt_plan** out, int family, int precision, int ndim, const int64_t* di
         }
         if (precision == JT_F64) build<double>(p.get());
         else build<float>(p.get());
+        JT_HIP(hipEventCreateWithFlags(&p->chain, hipEventDisableTiming));
         *out = p.release();
         return JT_OK;
     });
+}
+
+// Does `op` touch the plan's device buffers (beyond the tables, which are
+// read-only)?  Then calls on different streams are ordered (see jt_plan).
+bool touches_plan_buffers(const jt_plan* p, int op, bool staged) {
+    return staged || p->family == FAM_RAGGED || p->op_ws[op] > 0;
+}
+
+// The plan's workspace of `op`, and its staging area after it.
+char* ws_base(jt_plan* p) { return static_cast<char*>(p->precision == JT_F64 ? p->d->ws.p : p->f->ws.p); }
+char* staging_area(jt_plan* p, int op) {
+    const size_t cs = p->precision == JT_F64 ? 16 : 8;
+    return ws_base(p) + (size_t)p->op_ws[op] * cs;
+}
+
+// Enqueues `op` on device memory; the caller holds p->mu.
+int execute_locked(jt_plan* p, int op, char* ptr, int scale, hipStream_t st) {
+    const int64_t ne = op_elements(p, op);
+    if (ne < 0 || p->op_ws[op] < 0) return set_err(JT_BAD_ARG, "op not defined for this plan");
+    const size_t es = p->precision == JT_F64 ? 8 : 4;
+    // The kernels access (re, im) pairs of 2*es bytes: an offset that
+    // misaligns them (the reference accepts any offa) runs on an aligned copy
+    // of the range in the plan's staging area, on the same stream.
+    const bool staged = reinterpret_cast<uintptr_t>(ptr) % (2 * es) != 0;
+    bool chained = touches_plan_buffers(p, op, staged);
+    if (chained) {
+        // under the caller's stream capture the captured graph orders the
+        // work; an event recorded outside it cannot be waited for there
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        JT_HIP(hipStreamIsCapturing(st, &cs));
+        if (cs != hipStreamCaptureStatusNone) chained = false;
+    }
+    if (chained && p->chain_set && p->chain_st != st) JT_HIP(hipStreamWaitEvent(st, p->chain, 0));
+    const size_t bytes = (size_t)ne * es;
+    char* d = ptr;
+    if (staged) {
+        d = staging_area(p, op);
+        JT_HIP(hipMemcpyAsync(d, ptr, bytes, hipMemcpyDeviceToDevice, st));
+    }
+    int rc;
+    if (p->precision == JT_F64)
+        rc = exec_t<double>(p, op, reinterpret_cast<double*>(d), scale != 0,
+                            reinterpret_cast<cpx<double>*>(ws_base(p)), st);
+    else
+        rc = exec_t<float>(p, op, reinterpret_cast<float*>(d), scale != 0,
+                           reinterpret_cast<cpx<float>*>(ws_base(p)), st);
+    if (rc == JT_OK && staged) JT_HIP(hipMemcpyAsync(ptr, d, bytes, hipMemcpyDeviceToDevice, st));
+    if (chained) {
+        JT_HIP(hipEventRecord(p->chain, st));
+        p->chain_st = st;
+        p->chain_set = true;
+    }
+    return rc;
+}
+
+// Reads and clears the ragged batch's failure flag (set by a dependency wait
+// that timed out in any launch since the last check).
+int take_device_status(jt_plan* p) {
+    if (p->family != FAM_RAGGED) return JT_OK;
+    DevBuf& d = p->precision == JT_F64 ? p->d->ragged->d_err : p->f->ragged->d_err;
+    int e = 0;
+    JT_HIP(hipMemcpy(&e, d.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (!e) return JT_OK;
+    const int zero = 0;
+    JT_HIP(hipMemcpy(d.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+    return set_err(JT_HIP_ERROR, "a ragged-batch wait timed out; results are incomplete");
 }
 
 }  // namespace
@@ -143,13 +217,20 @@ int jt_plan_create_ragged(jt_plan** out, int precision, int64_t count, const int
         p->dims[0] = count;
         p->flags = flags;
         const bool tune = (flags & JT_FLAG_AUTOTUNE) != 0;
+        int64_t pts = 0;
         if (precision == JT_F64) {
             p->d = std::make_unique<Impl<double>>();
             p->d->ragged = make_ragged_tuned<double>(lengths, count, tune, &p->desc);
+            pts = p->d->ragged->total;
+            p->d->ws.alloc((size_t)std::max<int64_t>(pts, 1) * sizeof(cpx<double>));  // staging
         } else {
             p->f = std::make_unique<Impl<float>>();
             p->f->ragged = make_ragged_tuned<float>(lengths, count, tune, &p->desc);
+            pts = p->f->ragged->total;
+            p->f->ws.alloc((size_t)std::max<int64_t>(pts, 1) * sizeof(cpx<float>));  // staging
         }
+        p->op_ws[JT_COMPLEX_FORWARD] = p->op_ws[JT_COMPLEX_INVERSE] = 0;
+        JT_HIP(hipEventCreateWithFlags(&p->chain, hipEventDisableTiming));
         *out = p.release();
         return JT_OK;
     });
@@ -177,13 +258,8 @@ int jt_plan_sync_status(jt_plan* p) {
     return guarded([&]() -> int {
         if (!p) return set_err(JT_BAD_ARG, "null plan");
         JT_HIP(hipDeviceSynchronize());
-        if (p->family == FAM_RAGGED) {
-            int e = 0;
-            DevBuf& d = p->precision == JT_F64 ? p->d->ragged->d_err : p->f->ragged->d_err;
-            JT_HIP(hipMemcpy(&e, d.p, sizeof(int), hipMemcpyDeviceToHost));
-            if (e) return set_err(JT_HIP_ERROR, "a ragged-batch wait timed out; results are incomplete");
-        }
-        return JT_OK;
+        std::lock_guard<std::mutex> lk(p->mu);
+        return take_device_status(p);
     });
 }
 
@@ -195,6 +271,7 @@ int jt_plan_set_flags(jt_plan* p, uint32_t flags) {
             return set_err(JT_BAD_ARG, "split-FP16 form is defined for FP32 plans only");
         if (p->family != FAM_FFT) return set_err(JT_BAD_ARG, "flags apply to FFT plans");
         if (!p->f) return set_err(JT_BAD_ARG, "split-FP16 form is defined for FP32 plans only");
+        std::lock_guard<std::mutex> lk(p->mu);
         p->flags = flags;
         apply_flags(*p->f, flags);
         return JT_OK;
@@ -210,51 +287,18 @@ int jt_plan_destroy(jt_plan* plan) {
 
 int64_t jt_plan_elements(const jt_plan* p, int op) {
     if (!p) return -1;
-    if (p->family == FAM_RAGGED) {
-        int64_t t = p->precision == JT_F64 ? p->d->ragged->total : p->f->ragged->total;
-        return (op == JT_COMPLEX_FORWARD || op == JT_COMPLEX_INVERSE) ? 2 * t : -1;
-    }
-    const int64_t tot = total(p) * p->batch;
-    if (p->family != FAM_FFT) return tot;
-    switch (op) {
-        case JT_COMPLEX_FORWARD:
-        case JT_COMPLEX_INVERSE:
-        case JT_REAL_FORWARD_FULL:
-        case JT_REAL_INVERSE_FULL:
-            return 2 * tot;
-        case JT_REAL_FORWARD:
-        case JT_REAL_INVERSE:
-            return tot;
-        default:
-            return -1;
-    }
+    return op_elements(p, op);
 }
 
 int jt_execute(jt_plan* p, int op, void* data, int64_t offset, int scale, void* stream) {
     return guarded([&]() -> int {
         if (!p || !data) return set_err(JT_BAD_ARG, "null argument");
         if (offset < 0) return set_err(JT_BAD_ARG, "negative offset");
-        const int64_t ne = jt_plan_elements(p, op);
-        if (ne < 0) return set_err(JT_BAD_ARG, "op not defined for this plan");
-        hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+        if (op_elements(p, op) < 0) return set_err(JT_BAD_ARG, "op not defined for this plan");
         const size_t es = p->precision == JT_F64 ? 8 : 4;
-        char* ptr = static_cast<char*>(data) + offset * (int64_t)es;
-        auto run = [&](void* d) -> int {
-            if (p->precision == JT_F64) return exec_t<double>(p, op, static_cast<double*>(d), scale != 0, st);
-            return exec_t<float>(p, op, static_cast<float*>(d), scale != 0, st);
-        };
-        if (reinterpret_cast<uintptr_t>(ptr) % (2 * es) == 0) return run(ptr);
-        // The kernels access (re, im) pairs of 2*es bytes: an odd offset (the
-        // reference accepts any offa) runs on an aligned copy of the range,
-        // staged on the same stream.
-        DevBuf& stg = p->precision == JT_F64 ? p->d->staging : p->f->staging;
-        const size_t bytes = (size_t)ne * es;
-        if (stg.bytes < bytes) stg.alloc(bytes);
-        JT_HIP(hipMemcpyAsync(stg.p, ptr, bytes, hipMemcpyDeviceToDevice, st));
-        const int rc = run(stg.p);
-        if (rc != JT_OK) return rc;
-        JT_HIP(hipMemcpyAsync(ptr, stg.p, bytes, hipMemcpyDeviceToDevice, st));
-        return JT_OK;
+        std::lock_guard<std::mutex> lk(p->mu);
+        return execute_locked(p, op, static_cast<char*>(data) + offset * (int64_t)es, scale,
+                              reinterpret_cast<hipStream_t>(stream));
     });
 }
 
@@ -262,19 +306,23 @@ int jt_execute_host(jt_plan* p, int op, void* data, int64_t offset, int scale) {
     return guarded([&]() -> int {
         if (!p || !data) return set_err(JT_BAD_ARG, "null argument");
         if (offset < 0) return set_err(JT_BAD_ARG, "negative offset");
-        int64_t ne = jt_plan_elements(p, op);
-        if (ne < 0) return set_err(JT_BAD_ARG, "op not defined for this plan");
-        size_t es = p->precision == JT_F64 ? 8 : 4;
-        size_t bytes = (size_t)ne * es;
-        DevBuf& stg = p->precision == JT_F64 ? p->d->staging : p->f->staging;
-        if (stg.bytes < bytes) stg.alloc(bytes);
-        char* h = static_cast<char*>(data) + offset * es;
-        JT_HIP(hipMemcpy(stg.p, h, bytes, hipMemcpyHostToDevice));
-        int rc = jt_execute(p, op, stg.p, 0, scale, nullptr);
+        const int64_t ne = op_elements(p, op);
+        if (ne < 0 || p->op_ws[op] < 0) return set_err(JT_BAD_ARG, "op not defined for this plan");
+        const size_t es = p->precision == JT_F64 ? 8 : 4;
+        const size_t bytes = (size_t)ne * es;
+        char* h = static_cast<char*>(data) + offset * (int64_t)es;
+        // The whole call holds the plan: its staging area is the plan's.
+        std::lock_guard<std::mutex> lk(p->mu);
+        if (p->chain_set) JT_HIP(hipEventSynchronize(p->chain));  // earlier device calls are done
+        char* d = staging_area(p, op);
+        hipStream_t st = nullptr;
+        JT_HIP(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+        int rc = execute_locked(p, op, d, scale, st);
         if (rc != JT_OK) return rc;
-        rc = jt_plan_sync_status(p);
+        JT_HIP(hipStreamSynchronize(st));
+        rc = take_device_status(p);
         if (rc != JT_OK) return rc;
-        JT_HIP(hipMemcpy(h, stg.p, bytes, hipMemcpyDeviceToHost));
+        JT_HIP(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
         return JT_OK;
     });
 }

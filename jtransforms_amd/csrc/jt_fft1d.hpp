@@ -11,6 +11,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <memory>
 #include <stdexcept>
@@ -544,11 +545,15 @@ void go_gen(const Fft1D<T>& p, IO io, hipStream_t st) {
     size_t shm = 4 * (size_t)np * sizeof(T);
     int nt = n / 4;
     nt = nt < 64 ? 64 : (nt > 256 ? 256 : ((nt + 63) / 64) * 64);
-    static bool attr_set = false;  // per instantiation
-    if (!attr_set) {
+    // the LDS limit is raised once per instantiation and device
+    static std::atomic<uint64_t> attr_devs{0};
+    int dev = 0;
+    JT_HIP(hipGetDevice(&dev));
+    const uint64_t bit = dev < 64 ? (uint64_t(1) << dev) : 0;
+    if (!bit || !(attr_devs.load(std::memory_order_relaxed) & bit)) {
         JT_HIP(hipFuncSetAttribute((const void*)generic_kernel<T, DIR, IO>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_set = true;
+        attr_devs.fetch_or(bit);
     }
     int64_t lines = io.nlines;
     JT_PROF(st, std::string("generic<") + (sizeof(T) == 8 ? "f64" : "f32") + ",N=" + std::to_string(n) +

@@ -8,27 +8,43 @@
 
 namespace jt_api {
 
-// workspace requirement (complex elements)
+// Workspace (complex elements) that op touches; -1 if the op is not defined.
+// Mirrors the buffer uses of the execution paths below.
 template <typename T>
-int64_t ws_need(jt_plan* p) {
+int64_t op_ws_need(jt_plan* p, int op) {
     Impl<T>& I = impl<T>(p);
-    int64_t w = 0;
+    if (op_elements(p, op) < 0) return -1;
+    if (p->family == FAM_RAGGED) return 0;  // the batch's buffers live in its RaggedPlan
     const int64_t tot = total(p) * p->batch;
-    for (int a = 0; a < p->ndim; a++) {
-        if (!I.ax[a]) continue;
-        // (real-input multi-D passes run the column axes on half-width data,
-        // which needs no more than the complex passes)
-        int64_t lines = tot / p->dims[a];
-        w = std::max(w, axis_ws(*I.ax[a], lines));
+    if (p->family != FAM_FFT) {
+        int64_t w = 0;
+        for (int a = 0; a < p->ndim; a++) w = std::max(w, I.r2r[a]->ws_elems(tot / p->dims[a]));
+        return w;
     }
-    if (I.real) {
-        int64_t lines = tot / p->dims[p->ndim - 1];
-        w = std::max(w, I.real->ws_elems(lines));
-        w = std::max(w, 2 * tot);  // *Full ops expand to complex
+    // complex passes over every axis (run_complex_nd)
+    int64_t wc = 0;
+    if (p->ndim == 1) wc = I.ax[0]->workspace(p->batch);
+    else
+        for (int a = 0; a < p->ndim; a++) wc = std::max(wc, axis_ws(*I.ax[a], tot / p->dims[a]));
+    switch (op) {
+        case JT_COMPLEX_FORWARD:
+        case JT_COMPLEX_INVERSE:
+            return wc;
+        case JT_REAL_FORWARD:
+        case JT_REAL_INVERSE: {
+            const Real1D<T>& r = *I.real;
+            const int64_t lines = tot / p->dims[p->ndim - 1];
+            // even n: half-length complex plan; odd n: full complex copy
+            int64_t w = r.half ? r.half->workspace(lines) : lines * r.n + r.full->workspace(lines);
+            // multi-D: column passes over half-width planes (within wc)
+            if (p->ndim > 1) w = std::max(w, wc);
+            return w;
+        }
+        default:  // *Full: reals expanded to complex in the workspace
+            if (p->ndim == 1)  // (one line: batch plans reject the *Full ops)
+                return p->batch > 1 ? 0 : I.real->n + I.real->full->workspace(1);
+            return std::max(tot, wc);
     }
-    for (int a = 0; a < 3; a++)
-        if (I.r2r[a]) w = std::max(w, I.r2r[a]->ws_elems(tot / p->dims[a]) + tot);
-    return w;
 }
 
 // split-FP16 MFMA radix-16 stages on every compile-time complex axis plan
@@ -49,32 +65,39 @@ void build(jt_plan* p) {
     auto I = std::make_unique<Impl<T>>();
     if (p->family == FAM_FFT) {
         for (int a = 0; a < p->ndim; a++) {
-            bool dup = false;
-            for (int b = 0; b < a; b++)
-                if (p->dims[b] == p->dims[a]) dup = true;
-            (void)dup;
-            I->ax[a] = make_fft1d<T>(p->dims[a]);
+            for (int b = 0; b < a && !I->ax[a]; b++)
+                if (p->dims[b] == p->dims[a]) I->ax[a] = I->ax[b];
+            if (!I->ax[a]) I->ax[a] = make_fft1d<T>(p->dims[a]);
         }
         // packed real transforms run the last axis as a real plan; the other
         // axes of a multi-D real transform are complex passes of full length
         I->real = make_real1d<T>(p->dims[p->ndim - 1]);
     } else {
-        for (int a = 0; a < p->ndim; a++) I->r2r[a] = make_r2r1d<T>(p->family, p->dims[a]);
+        for (int a = 0; a < p->ndim; a++) {
+            for (int b = 0; b < a && !I->r2r[a]; b++)
+                if (p->dims[b] == p->dims[a]) I->r2r[a] = I->r2r[b];
+            if (!I->r2r[a]) I->r2r[a] = make_r2r1d<T>(p->family, p->dims[a]);
+        }
     }
     apply_flags(*I, p->flags);
     if constexpr (sizeof(T) == 8) p->d = std::move(I);
     else p->f = std::move(I);
-    int64_t w = ws_need<T>(p);
-    impl<T>(p).ws.alloc((size_t)std::max<int64_t>(w, 1) * sizeof(cpx<T>));
+    // one buffer: each op's workspace followed by its staging area
+    int64_t w = 1;
+    for (int op = 0; op < NUM_OPS; op++) {
+        p->op_ws[op] = op_ws_need<T>(p, op);
+        if (p->op_ws[op] < 0) continue;
+        w = std::max(w, p->op_ws[op] + (op_elements(p, op) + 1) / 2);
+    }
+    impl<T>(p).ws.alloc((size_t)w * sizeof(cpx<T>));
 }
 
 // ---------------------------------------------------------------------------
 // execution
 // ---------------------------------------------------------------------------
 template <typename T, int DIR>
-void run_complex_nd(jt_plan* p, cpx<T>* a, bool scale, hipStream_t st) {
+void run_complex_nd(jt_plan* p, cpx<T>* a, bool scale, cpx<T>* ws, hipStream_t st) {
     Impl<T>& I = impl<T>(p);
-    cpx<T>* ws = I.ws.template as<cpx<T>>();
     const int nd = p->ndim;
     const int64_t B = p->batch;
     const T s = (DIR > 0 && scale) ? T(1.0 / (double)total(p)) : T(1);
@@ -99,9 +122,8 @@ void run_complex_nd(jt_plan* p, cpx<T>* a, bool scale, hipStream_t st) {
 
 // packed real 2D/3D (power-of-two sizes)
 template <typename T>
-void run_real_forward_nd(jt_plan* p, T* a, hipStream_t st) {
+void run_real_forward_nd(jt_plan* p, T* a, cpx<T>* ws, hipStream_t st) {
     Impl<T>& I = impl<T>(p);
-    cpx<T>* ws = I.ws.template as<cpx<T>>();
     cpx<T>* z = reinterpret_cast<cpx<T>*>(a);
     if (p->ndim == 2) {
         const int64_t r = p->dims[0], c = p->dims[1], ch = c / 2;
@@ -125,9 +147,8 @@ void run_real_forward_nd(jt_plan* p, T* a, hipStream_t st) {
 }
 
 template <typename T>
-void run_real_inverse_nd(jt_plan* p, T* a, bool scale, hipStream_t st) {
+void run_real_inverse_nd(jt_plan* p, T* a, bool scale, cpx<T>* ws, hipStream_t st) {
     Impl<T>& I = impl<T>(p);
-    cpx<T>* ws = I.ws.template as<cpx<T>>();
     cpx<T>* z = reinterpret_cast<cpx<T>*>(a);
     if (p->ndim == 2) {
         const int64_t r = p->dims[0], c = p->dims[1], ch = c / 2;
@@ -157,11 +178,9 @@ void run_real_inverse_nd(jt_plan* p, T* a, bool scale, hipStream_t st) {
 // realForwardFull / realInverseFull for 2D/3D: complex transform of the real
 // input (first prod(dims) elements), written as interleaved complex.
 template <typename T, int DIR>
-void run_real_full_nd(jt_plan* p, T* a, bool scale, hipStream_t st) {
-    Impl<T>& I = impl<T>(p);
+void run_real_full_nd(jt_plan* p, T* a, bool scale, cpx<T>* ws, hipStream_t st) {
     const int64_t tot = total(p);
-    // expand in place from the back: reals -> complex (use workspace copy)
-    cpx<T>* ws = I.ws.template as<cpx<T>>();
+    // reals -> complex through the workspace, then the complex transform
     { JT_PROF(st, "real_to_complex");
     hipLaunchKernelGGL((real_to_complex<T>), ew_grid(tot), dim3(256), 0, st, a, ws, tot, (int64_t)1);
     }
@@ -171,13 +190,12 @@ void run_real_full_nd(jt_plan* p, T* a, bool scale, hipStream_t st) {
                        tot);
     }
     JT_HIP(hipGetLastError());
-    run_complex_nd<T, DIR>(p, reinterpret_cast<cpx<T>*>(a), scale, st);
+    run_complex_nd<T, DIR>(p, reinterpret_cast<cpx<T>*>(a), scale, ws, st);
 }
 
 template <typename T>
-int exec_t(jt_plan* p, int op, T* a, bool scale, hipStream_t st) {
+int exec_t(jt_plan* p, int op, T* a, bool scale, cpx<T>* ws, hipStream_t st) {
     Impl<T>& I = impl<T>(p);
-    cpx<T>* ws = I.ws.template as<cpx<T>>();
     if (p->family == FAM_RAGGED) {
         if (op != JT_COMPLEX_FORWARD && op != JT_COMPLEX_INVERSE)
             return set_err(JT_BAD_ARG, "a ragged batch supports complexForward / complexInverse");
@@ -195,11 +213,11 @@ int exec_t(jt_plan* p, int op, T* a, bool scale, hipStream_t st) {
     switch (op) {
         case JT_COMPLEX_FORWARD:
             if (nd == 1 && n == 1) return JT_OK;
-            run_complex_nd<T, -1>(p, reinterpret_cast<cpx<T>*>(a), false, st);
+            run_complex_nd<T, -1>(p, reinterpret_cast<cpx<T>*>(a), false, ws, st);
             return JT_OK;
         case JT_COMPLEX_INVERSE:
             if (nd == 1 && n == 1) return JT_OK;
-            run_complex_nd<T, 1>(p, reinterpret_cast<cpx<T>*>(a), scale, st);
+            run_complex_nd<T, 1>(p, reinterpret_cast<cpx<T>*>(a), scale, ws, st);
             return JT_OK;
         case JT_REAL_FORWARD:
             if (nd == 1) {
@@ -209,7 +227,7 @@ int exec_t(jt_plan* p, int op, T* a, bool scale, hipStream_t st) {
             if (!p->pow2_all)
                 return set_err(JT_NOT_POW2, nd == 2 ? "rows and columns must be power of two numbers"
                                                     : "slices, rows and columns must be power of two numbers");
-            run_real_forward_nd<T>(p, a, st);
+            run_real_forward_nd<T>(p, a, ws, st);
             return JT_OK;
         case JT_REAL_INVERSE:
             if (nd == 1) {
@@ -219,7 +237,7 @@ int exec_t(jt_plan* p, int op, T* a, bool scale, hipStream_t st) {
             if (!p->pow2_all)
                 return set_err(JT_NOT_POW2, nd == 2 ? "rows and columns must be power of two numbers"
                                                     : "slices, rows and columns must be power of two numbers");
-            run_real_inverse_nd<T>(p, a, scale, st);
+            run_real_inverse_nd<T>(p, a, scale, ws, st);
             return JT_OK;
         case JT_REAL_FORWARD_FULL:
             if (nd == 1) {
@@ -227,7 +245,7 @@ int exec_t(jt_plan* p, int op, T* a, bool scale, hipStream_t st) {
                 real_full<T, -1>(*I.real, a, 1, false, ws, st);
                 return JT_OK;
             }
-            run_real_full_nd<T, -1>(p, a, false, st);
+            run_real_full_nd<T, -1>(p, a, false, ws, st);
             return JT_OK;
         case JT_REAL_INVERSE_FULL:
             if (nd == 1) {
@@ -235,7 +253,7 @@ int exec_t(jt_plan* p, int op, T* a, bool scale, hipStream_t st) {
                 real_full<T, 1>(*I.real, a, 1, scale, ws, st);
                 return JT_OK;
             }
-            run_real_full_nd<T, 1>(p, a, scale, st);
+            run_real_full_nd<T, 1>(p, a, scale, ws, st);
             return JT_OK;
         default:
             return set_err(JT_BAD_ARG, "op not defined for this plan");

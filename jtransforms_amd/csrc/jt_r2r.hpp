@@ -433,7 +433,7 @@ void r2r_axis(int family, const R2R1D<T>& r, T* data, LineMap lm, bool fwd, bool
 }
 
 template <typename T>
-void run_r2r_nd(int family, int nd, const int64_t* d, const std::unique_ptr<R2R1D<T>>* r, T* a, bool fwd,
+void run_r2r_nd(int family, int nd, const int64_t* d, const std::shared_ptr<R2R1D<T>>* r, T* a, bool fwd,
                 bool scale, cpx<T>* ws, hipStream_t st) {
     if (nd == 1) {
         r2r_axis<T>(family, *r[0], a, LineMap{1, 1, d[0], 0, 1}, fwd, scale, ws, st);

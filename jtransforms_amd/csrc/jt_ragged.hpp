@@ -340,10 +340,10 @@ template <typename T>
 __global__ void ragged_reset(RArgs<T> a) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     for (; i < 3 * a.ntr; i += (int64_t)gridDim.x * blockDim.x) a.cnt[i] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        *a.next = 0;
-        *a.err = 0;
-    }
+    // (the failure flag is not cleared here: it accumulates over launches
+    // until the host reads and clears it, so a timeout is never lost to a
+    // later clean launch)
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.next = 0;
 }
 
 template <typename T, int NTMAX>

@@ -41,6 +41,10 @@ def test_version_and_status_strings():
     (jt.DoubleFFT_3D, (4, 1, 4), "slices, rows and columns must be greater than 1"),
     (jt.DoubleDCT_1D, (0,), "n must be greater than 0"),
     (jt.DoubleDCT_1D, ((1 << 28) + 1,), "n must be smaller or equal to 268435456"),  # DoubleDCT_1D.java:98-100
+    (jt.FloatDCT_1D, ((1 << 31) - 1,), "when useLargeArrays argument is set to false"),
+    # DoubleDST_1D builds a DoubleDCT_1D(n) (DoubleDST_1D.java:59-65)
+    (jt.DoubleDST_1D, ((1 << 28) + 1,), "n must be smaller or equal to 268435456"),
+    (jt.FloatDST_1D, ((1 << 30),), "n must be smaller or equal to 268435456"),
 ])
 def test_illegal_argument(ctor, args, msg):
     with pytest.raises(jt.IllegalArgumentException, match=re.escape(msg)):

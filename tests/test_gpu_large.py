@@ -31,7 +31,9 @@ def _need_gib(gib):
 def test_1d_impulse_beyond_2_31_points():
     import torch
     n = 1 << 31  # 2^31 complex points: 2^32 doubles, 32 GiB
-    _need_gib(104)  # data 32 GiB + four-step workspace 64 GiB
+    # data 32 GiB; the plan's buffer holds the largest op's workspace plus its
+    # staging area: realForwardFull, 96 + 32 GiB
+    _need_gib(168)
     p = n - 123457
     plan = jt.DoubleFFT_1D(n)
     print(f"DoubleFFT_1D({n}) plan: {plan.describe()}", flush=True)
@@ -61,7 +63,7 @@ def test_1d_impulse_beyond_2_31_points():
 def test_batch_beyond_2_31_points(orc):
     import torch
     n, lines = 4096, (1 << 19) + 8  # 2^31 + 32768 complex points, 32 GiB
-    _need_gib(40)
+    _need_gib(72)  # data 32 GiB + the plan's staging area 32 GiB
     plan = jt.FFTBatch1D(n, lines, precision="f64")
     x = torch.empty(2 * n * lines, dtype=torch.float64, device="cuda")
     g = torch.Generator(device="cuda")

@@ -110,6 +110,14 @@ def test_fft3d_real(orc, dims, prec):
     b = a.copy()
     p.realInverse(b, True)
     assert rel(b, x) <= 2 * tol(tot, prec)
+    # unscaled realInverse (DoubleFFT_3D.java:1629-1645) against the oracle,
+    # whose unscaled factor is pinned by the reference's call sequence
+    u = a.copy()
+    p.realInverse(u, False)
+    want = orc.fft3d_real_inverse(s, r, c, a.astype(np.float64), False)
+    e = rel(u, want)
+    print(f"{prec} 3D {dims} realInverse(scale=false) rel={e:.3e}", flush=True)
+    assert e <= 2 * tol(tot, prec)
 
 
 R2R = {

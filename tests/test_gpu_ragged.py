@@ -130,7 +130,8 @@ def test_ragged_autotune(orc, prec):
 def test_ragged_wait_timeout_is_reported(orc, monkeypatch):
     """Failure path of the bounded dependency waits, forced from the host:
     every consumer item waits for a count no producer reaches, gives up after
-    1 ms, and the plan reports the failure; the next launch is clean."""
+    1 ms, and the plan reports the failure, also when a clean launch ran
+    after the failed one; once reported, the next launch is clean."""
     import torch
     lens = np.array([12800, 65536, 12800], dtype=np.int64)   # four-step items
     p = jt.RaggedBatch1D(lens, precision="f64")
@@ -140,7 +141,13 @@ def test_ragged_wait_timeout_is_reported(orc, monkeypatch):
     p.complexForward(t)
     with pytest.raises(jt.JTransformsError, match="timed out"):
         p.sync_status()
+    # the flag is sticky: a clean launch queued before the check does not
+    # hide the earlier failure
+    p.complexForward(t)
     monkeypatch.delenv("JT_RAGGED_FORCE_TIMEOUT")
+    p.complexForward(t)
+    with pytest.raises(jt.JTransformsError, match="timed out"):
+        p.sync_status()
     t.copy_(torch.from_numpy(x))
     p.complexForward(t)
     p.sync_status()

@@ -171,6 +171,67 @@ def test_fft3d_real_forward_layout(orc, dims):
     assert orc.rmse(un / (s * r * c / 2), x) <= EPS
 
 
+# Unscaled 2D/3D realInverse, pinned by following the reference's call
+# sequence with independently pinned 1D pieces (no constant restated here):
+#   DoubleFFT_2D.realInverse (fft/DoubleFFT_2D.java:1077-1095):
+#       rdft2d_sub(-1); cdft2d_sub(1, scale) over the complex columns of the
+#       packed rows; fftColumns.realInverse(row, scale) for every row
+#   DoubleFFT_3D.realInverse (fft/DoubleFFT_3D.java:1629-1645):
+#       rdft3d_sub(-1); cdft3db_sub(1, scale) along slices; xdft3da_sub1(1, 1):
+#       complexInverse along rows, then realInverse of every row
+# The forward path ends with rdft2d_sub(+1) / rdft3d_sub(+1), pairwise maps
+# (u, v) -> ((u+v)/2, (u-v)/2) whose (-1) forms (u, v) -> (u+v, u-v) are their
+# exact inverses (DoubleFFT_2D.java:2544-2574, DoubleFFT_3D.java:6909-7021),
+# so the state the inverse's column passes see is the forward state before
+# that fix: the row realForward followed by the column complexForward passes.
+# The 1D pieces are pinned above: real_forward / real_inverse(scale=false) by
+# numpy and DoubleFFT_1DTest.java:583-588, complex_forward / complex_inverse
+# (scale=false) by the FFTW fixtures and numpy.
+def _rows_real(orc, fn, x, cols, *extra):
+    m = x.reshape(-1, cols)
+    return np.stack([fn(cols, m[i].copy(), *extra) for i in range(m.shape[0])])
+
+
+def _complex_axis(orc, fn, z, axis, *extra):
+    z = np.moveaxis(z, axis, -1)
+    n = z.shape[-1]
+    flat = z.reshape(-1, n)
+    out = np.empty_like(flat)
+    for i in range(flat.shape[0]):
+        out[i] = c2n(fn(n, n2c(flat[i]), *extra))
+    return np.moveaxis(out.reshape(z.shape), -1, axis)
+
+
+@pytest.mark.parametrize("rows,cols", [(4, 8), (8, 16), (16, 4), (32, 32)])
+def test_fft2d_real_inverse_unscaled_by_call_sequence(orc, rows, cols):
+    x = orc.jrandom_doubles(SEED, rows * cols)
+    packed = orc.fft2d_real_forward(rows, cols, x)
+    # forward state before rdft2d_sub(+1)
+    z = c2n(_rows_real(orc, orc.real_forward, x, cols).ravel()).reshape(rows, cols // 2)
+    y = _complex_axis(orc, orc.complex_forward, z, 0)
+    # the reference's inverse sequence after rdft2d_sub(-1), unscaled
+    u = _complex_axis(orc, orc.complex_inverse, y, 0, False)
+    want = _rows_real(orc, orc.real_inverse, n2c(u.ravel()), cols, False).ravel()
+    got = orc.fft2d_real_inverse(rows, cols, packed, False)
+    assert orc.rel_l2(got, want) <= 1e-14
+    assert orc.rel_l2(orc.fft2d_real_inverse(rows, cols, packed, True), x) <= 1e-14
+
+
+@pytest.mark.parametrize("dims", [(2, 4, 8), (4, 8, 4), (8, 4, 16), (16, 16, 16)])
+def test_fft3d_real_inverse_unscaled_by_call_sequence(orc, dims):
+    s, r, c = dims
+    x = orc.jrandom_doubles(20110625, s * r * c)
+    packed = orc.fft3d_real_forward(s, r, c, x)
+    z = c2n(_rows_real(orc, orc.real_forward, x, c).ravel()).reshape(s, r, c // 2)
+    y = _complex_axis(orc, orc.complex_forward, _complex_axis(orc, orc.complex_forward, z, 1), 0)
+    u = _complex_axis(orc, orc.complex_inverse, y, 0, False)
+    u = _complex_axis(orc, orc.complex_inverse, u, 1, False)
+    want = _rows_real(orc, orc.real_inverse, n2c(u.ravel()), c, False).ravel()
+    got = orc.fft3d_real_inverse(s, r, c, packed, False)
+    assert orc.rel_l2(got, want) <= 1e-14
+    assert orc.rel_l2(orc.fft3d_real_inverse(s, r, c, packed, True), x) <= 1e-14
+
+
 @pytest.mark.parametrize("dims", [(2, 3, 4), (8, 8, 8), (5, 6, 7)])
 def test_fft3d_complex(orc, dims):
     x = orc.jrandom_doubles(20110625, 2 * int(np.prod(dims)), signed=False)
