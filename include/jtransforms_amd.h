@@ -149,6 +149,17 @@ int jt_plan_destroy(jt_plan* plan);
  * the method's boolean argument where it has one and is ignored otherwise. */
 int jt_execute(jt_plan* plan, int op, void* data, int64_t offset, int scale, void* stream);
 
+/* Same as jt_execute, allocating nothing: the op's workspace (four-step,
+ * Bluestein and *Full intermediates) and the staging of a misaligned offset
+ * live in a static device array of 64 MiB per GPU (held until the process
+ * ends) instead of the plan's own buffer.  No counterpart in the reference;
+ * for callers that cannot allocate device memory, e.g. code under HIP graph
+ * capture or a serving loop.  An op that needs more than 64 MiB of scratch
+ * returns JT_TOO_BIG; ragged batches return JT_UNSUPPORTED.  All calls on one
+ * GPU share its array: calls that overlap in time on one GPU (other streams,
+ * other threads) race, so the caller orders them. */
+int jt_execute_noalloc(jt_plan* plan, int op, void* data, int64_t offset, int scale, void* stream);
+
 /* Same on a host array (the Java double[]/float[] a JNI binding pins with
  * GetPrimitiveArrayCritical): copies the touched range to the device, runs,
  * copies back, and returns when the result is in `data`. */

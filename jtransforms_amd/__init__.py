@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = [
     "jt_plan_create_dst", "jt_plan_create_dht", "jt_plan_create_fft1d_batch", "jt_plan_destroy",
     "jt_execute", "jt_execute_host", "jt_plan_elements", "jt_status_string", "jt_last_error",
     "jt_version", "jt_profile_begin", "jt_profile_end", "jt_plan_set_flags",
-    "jt_plan_create_ragged", "jt_plan_sync_status", "jt_plan_describe",
+    "jt_plan_create_ragged", "jt_plan_sync_status", "jt_plan_describe", "jt_execute_noalloc",
 ]
 
 
@@ -92,6 +92,7 @@ def lib():
                                  ctypes.c_void_p]
         L.jt_execute_host.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, i64,
                                       ctypes.c_int]
+        L.jt_execute_noalloc.argtypes = L.jt_execute.argtypes
         L.jt_plan_elements.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.jt_plan_elements.restype = i64
         L.jt_status_string.argtypes = [ctypes.c_int]

@@ -25,7 +25,33 @@ int set_err(int code, const std::string& msg) {
 
 using namespace jt_api;
 
+// Device scratch of jt_execute_noalloc: one array per GPU, loaded with the
+// code object and held until the process ends.  Host code never uses the
+// array's name as an address (on the host it is a stand-in symbol): the
+// device address comes from hipGetSymbolAddress, once per device.
+constexpr size_t JT_NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[JT_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// This device's copy of `scratch`, looked up on its first use.
+char* scratch_on_current_device() {
+    int dev = 0;
+    JT_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) throw JtError(ST_UNSUPPORTED, "device ordinal beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        JT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw JtError(ST_HIP, "hipGetSymbolAddress(scratch) returned no address");
+        g_scratch_ptr[dev] = ptr;
+    }
+    return static_cast<char*>(g_scratch_ptr[dev]);
+}
 
 template <typename F>
 int guarded(F&& fn) {
@@ -299,6 +325,43 @@ int jt_execute(jt_plan* p, int op, void* data, int64_t offset, int scale, void* 
         std::lock_guard<std::mutex> lk(p->mu);
         return execute_locked(p, op, static_cast<char*>(data) + offset * (int64_t)es, scale,
                               reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+int jt_execute_noalloc(jt_plan* p, int op, void* data, int64_t offset, int scale, void* stream) {
+    return guarded([&]() -> int {
+        if (!p || !data) return set_err(JT_BAD_ARG, "null argument");
+        if (offset < 0) return set_err(JT_BAD_ARG, "negative offset");
+        const int64_t ne = op_elements(p, op);
+        if (ne < 0 || p->op_ws[op] < 0) return set_err(JT_BAD_ARG, "op not defined for this plan");
+        if (p->family == FAM_RAGGED)
+            return set_err(JT_UNSUPPORTED, "ragged batches keep their work queue in the plan; use jt_execute");
+        const size_t es = p->precision == JT_F64 ? 8 : 4;
+        const size_t cs = 2 * es;
+        char* ptr = static_cast<char*>(data) + offset * (int64_t)es;
+        const bool staged = reinterpret_cast<uintptr_t>(ptr) % cs != 0;
+        const size_t bytes = (size_t)ne * es;
+        const size_t need = (size_t)p->op_ws[op] * cs + (staged ? (bytes + cs - 1) / cs * cs : 0);
+        if (need > JT_NOALLOC_SCRATCH_BYTES)
+            return set_err(JT_TOO_BIG, "jt_execute_noalloc: this op needs " + std::to_string(need) +
+                                           " bytes of scratch, more than its " +
+                                           std::to_string(JT_NOALLOC_SCRATCH_BYTES) + "-byte device array");
+        char* ws = scratch_on_current_device();
+        hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+        std::lock_guard<std::mutex> lk(p->mu);  // host-side plan state only
+        char* d = ptr;
+        if (staged) {
+            d = ws + (size_t)p->op_ws[op] * cs;
+            JT_HIP(hipMemcpyAsync(d, ptr, bytes, hipMemcpyDeviceToDevice, st));
+        }
+        int rc;
+        if (p->precision == JT_F64)
+            rc = exec_t<double>(p, op, reinterpret_cast<double*>(d), scale != 0, reinterpret_cast<cpx<double>*>(ws),
+                                st);
+        else
+            rc = exec_t<float>(p, op, reinterpret_cast<float*>(d), scale != 0, reinterpret_cast<cpx<float>*>(ws), st);
+        if (rc == JT_OK && staged) JT_HIP(hipMemcpyAsync(ptr, d, bytes, hipMemcpyDeviceToDevice, st));
+        return rc;
     });
 }
 
