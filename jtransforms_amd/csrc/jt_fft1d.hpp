@@ -164,14 +164,20 @@ std::vector<cpx<T>> stage_twiddles(const std::vector<int>& radices, std::vector<
 
 // Twiddles of the split-FP16 MFMA kernels (layout of mf_twoff): 16 factors
 // per jm (r = 0..15) for every radix-16 stage, R-1 for a closing radix R.
+// The radix-16 stages after the first carry the kernel's fixed 2^-4 range
+// step (exact): their inputs stay below 2^14 (jt_mfma.hpp).
 template <typename T>
 std::vector<cpx<T>> stage_twiddles_mf(const std::vector<int>& radices) {
     std::vector<cpx<T>> tw;
     int64_t ns = 1;
     for (int R : radices) {
         const int r0 = R == 16 ? 0 : 1;
+        const T f = (R == 16 && ns > 1) ? T(0.0625) : T(1);
         for (int64_t jm = 0; jm < ns; jm++)
-            for (int r = r0; r < R; r++) tw.push_back(expm<T>(jm * r, ns * R));
+            for (int r = r0; r < R; r++) {
+                cpx<T> w = expm<T>(jm * r, ns * R);
+                tw.push_back(cpx<T>{w.x * f, w.y * f});
+            }
         ns *= R;
     }
     return tw;

@@ -6,13 +6,17 @@
 //     X = F16 * V,  F16[i][k] = exp(DIR 2 pi i ik/16)
 // written over the reals as two 16x32 by 32x16 products:
 //     Xr = [Fr | -Fi] [Vr; Vi],   Xi = [Fi | Fr] [Vr; Vi]
-// with v_mfma_f32_16x16x32_f16.  Each operand is split as x ~ hi + lo
-// (hi = fp16(x), lo = fp16(x - hi)) and the product is the sum of the three
+// with v_mfma_f32_16x16x32_f16.  Each operand is split as x = hi + lo with
+// hi = x rounded toward zero to fp16's 11 significant bits (a bit mask on
+// the float) and lo = fp16(x - hi); the product is the sum of the three
 // terms hi*hi + hi*lo + lo*hi accumulated in FP32 (the lo*lo term, ~2^-22
-// relative, is dropped).  The data operand is scaled per butterfly (per MFMA
-// column) by a power of two that brings its largest entry to [2^13, 2^14),
-// so fp16's range holds it and the scaling is exact; results are unscaled
-// in FP32.
+// relative, is dropped).
+//
+// Range: the caller keeps every B entry below 2^14 in magnitude (block
+// floating point: one power-of-two scale per line chosen from the line's
+// input, then a fixed 2^-4 per radix-16 stage, folded into the twiddles, so
+// a stage's outputs, at most 16 times its inputs, stay below 2^14 too).  All
+// scalings are exact powers of two and are undone once at the store.
 //
 // Operand layouts of the 16x16x32 f16 MFMA (checked on the GPU): lane l
 // holds A[i][k] with i = l%16, k = 8*(l/16)+v; B[k][j] with j = l%16,
@@ -21,6 +25,19 @@
 #include "jt_dft.hpp"
 
 namespace jt {
+
+// Power-of-two exponent sh that brings a line whose largest |entry| is m
+// into [2^13, 2^14): m = f 2^e with f in [1, 2) gives sh = 13 - e, read from
+// the float's exponent bits; clamped so that 2^sh and the store's undo factor
+// 2^(4 (stages - 1) - sh) are normal floats.  m = 0 (or a subnormal m) gives
+// the clamp.
+__device__ __forceinline__ int mf_line_shift(unsigned mbits) {
+    const int e = (int)((mbits >> 23) & 0xFFu) - 127;
+    int sh = 13 - e;
+    sh = sh < -100 ? -100 : sh;
+    return sh > 100 ? 100 : sh;
+}
+__device__ __forceinline__ float mf_pow2(int k) { return __uint_as_float((unsigned)(k + 127) << 23); }
 
 typedef _Float16 jt_half8 __attribute__((ext_vector_type(8)));
 typedef float jt_float4 __attribute__((ext_vector_type(4)));
@@ -59,39 +76,28 @@ struct MfmaDft16 {
 
     // One 16-butterfly tile.  b: this lane's 8 entries of the B operand
     // (column lane%16, rows 8*(lane/16)+u: Re of points 8*((lane/16)&1)+u for
-    // lane/16 < 2, Im of them otherwise).  Returns this lane's 4 outputs
-    // i = 4*(lane/16)+u of butterfly lane%16, real parts in outr, imaginary
-    // parts in outi, in FP32.
+    // lane/16 < 2, Im of them otherwise), |b| < 2^14.  Returns this lane's 4
+    // outputs i = 4*(lane/16)+u of butterfly lane%16, real parts in outr,
+    // imaginary parts in outi, in FP32.
     __device__ __forceinline__ void tile(const float* b, float* outr, float* outi) const {
-        float m = 0.0f;
-#pragma unroll
-        for (int u = 0; u < 8; u++) m = fmaxf(m, fabsf(b[u]));
-        m = fmaxf(m, __shfl_xor(m, 16));
-        m = fmaxf(m, __shfl_xor(m, 32));
-        int ex = 0;
-        (void)frexpf(m, &ex);  // m = f 2^ex, f in [0.5, 1)
-        const int sh = (m > 0.0f) ? (14 - ex) : 0;
-        const float s = ldexpf(1.0f, sh);
-        const float inv = ldexpf(1.0f, -sh);
         jt_half8 bh, bl;
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-            float x = b[u] * s;
-            _Float16 h = (_Float16)x;
-            bh[u] = h;
-            bl[u] = (_Float16)(x - (float)h);
+            const float h = __uint_as_float(__float_as_uint(b[u]) & 0xFFFFE000u);  // exact in fp16
+            bh[u] = (_Float16)h;
+            bl[u] = (_Float16)(b[u] - h);
         }
         jt_float4 dr = {0.f, 0.f, 0.f, 0.f}, di = {0.f, 0.f, 0.f, 0.f};
         dr = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bh, dr, 0, 0, 0);
-        dr = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bl, dr, 0, 0, 0);
-        dr = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bh, dr, 0, 0, 0);
         di = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bh, di, 0, 0, 0);
+        dr = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bl, dr, 0, 0, 0);
         di = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bl, di, 0, 0, 0);
+        dr = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bh, dr, 0, 0, 0);
         di = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bh, di, 0, 0, 0);
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            outr[u] = dr[u] * inv;
-            outi[u] = di[u] * inv;
+            outr[u] = dr[u];
+            outi[u] = di[u];
         }
     }
 };

@@ -13,7 +13,12 @@
 //   * the product leaves output i = 4*(l/16)+u of that butterfly, complex, in
 //     lane l; the lane multiplies it by the NEXT radix-16 stage's twiddle for
 //     its destination and writes it to LDS (or, after the last stage of a
-//     contiguous line, straight to HBM).
+//     contiguous line, straight to HBM);
+//   * block floating point: the first stage's inputs of a line are scaled by
+//     one power of two from the line's largest entry (an LDS max over the
+//     workgroup's lines) into [2^13, 2^14); each later radix-16 stage's
+//     twiddles carry 2^-4, so every B operand stays below 2^14; the store
+//     undoes the product of these factors.  No per-tile range search.
 // So the twiddle never has to be applied in the B layout (where Re and Im of
 // a point sit in different lanes), and each stage costs one LDS write and one
 // LDS read per point, as in the FP32 kernel.  A radix 2/4/8 stage may close
@@ -37,6 +42,21 @@ __host__ __device__ constexpr int mf_twoff(int s) {
     return off;
 }
 
+// Output v (0..15) of radix-16 butterfly j at span ns (a power of 16) is point
+// p = a + v*ns with a = (j/ns)*16ns + j%ns, stored at lds_pos(p) = p + p/32.
+// The padding term splits into a per-butterfly part and a part linear in v
+// (so a lane computes one base per butterfly and adds constants):
+//   ns = 1:    p lies in one 32-point block: lds_pos(a) + v;
+//   ns = 16:   a%32 < 16, so p/32 = a/32 + v/2: lds_pos(a) + 16v + v/2;
+//   ns >= 256: ns is a multiple of 32: lds_pos(a) + v*(ns + ns/32).
+__device__ __forceinline__ int mf_slot_base(int j, int ns) {
+    const int a = (j / ns) * ns * 16 + (j % ns);
+    return a + (a >> 5);
+}
+__device__ __forceinline__ int mf_slot_step(int ns, int v) {
+    return ns == 1 ? v : (ns == 16 ? 16 * v + (v >> 1) : v * (ns + (ns >> 5)));
+}
+
 template <int DIR, int N, int W, bool COLMODE, class IO, int... Rs>
 __global__ void __launch_bounds__(N / 16 * W) mfma_stockham_kernel(IO io, const cpx<float>* __restrict__ tw) {
     using List = RadixList<Rs...>;
@@ -52,6 +72,7 @@ __global__ void __launch_bounds__(N / 16 * W) mfma_stockham_kernel(IO io, const 
     using Eng = Engine<float, DIR, N, TL, W, COLMODE, false>;
     __shared__ float lre[G::size];
     __shared__ float lim[G::size];
+    __shared__ unsigned lmax[W];  // per line: bits of the largest |input|
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -86,6 +107,7 @@ __global__ void __launch_bounds__(N / 16 * W) mfma_stockham_kernel(IO io, const 
     const bool eact = eline < io.nlines;
     const int64_t ebase = io.base(eact ? eline : gid * W);
 
+    if (tid < W) lmax[tid] = 0u;
     float b[4][8];
     // ---- inputs of the first stage in the B layout
     if constexpr (!COLMODE) {
@@ -118,6 +140,27 @@ __global__ void __launch_bounds__(N / 16 * W) mfma_stockham_kernel(IO io, const 
         }
     }
 
+    // ---- one power-of-two scale per line (block floating point)
+    if constexpr (!COLMODE) __syncthreads();  // lmax cleared
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float m = 0.0f;
+#pragma unroll
+        for (int u = 0; u < 8; u++) m = fmaxf(m, fabsf(b[q][u]));
+        atomicMax(&lmax[tl_line[q]], __float_as_uint(m));
+    }
+    __syncthreads();
+    constexpr int STEP = 4 * (NMF - 1);  // the later stages' 2^-4 factors
+    float undo[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int sh = mf_line_shift(lmax[tl_line[q]]);
+        const float sc = mf_pow2(sh);
+        undo[q] = io.scale * mf_pow2(STEP - sh);
+#pragma unroll
+        for (int u = 0; u < 8; u++) b[q][u] *= sc;
+    }
+
     // ---- the radix-16 stages
     int ns = 1;
 #pragma unroll
@@ -136,7 +179,7 @@ __global__ void __launch_bounds__(N / 16 * W) mfma_stockham_kernel(IO io, const 
 #pragma unroll
                     for (int u = 0; u < 4; u++)
                     {
-                        const cpx<float> o{outr[q][u] * io.scale, outi[q][u] * io.scale};
+                        const cpx<float> o{outr[q][u] * undo[q], outi[q][u] * undo[q]};
                         if constexpr ((IO::streaming & NT_STORE) != 0) st_nt(dst + TL * u, o);
                         else dst[TL * u] = o;
                     }
@@ -151,17 +194,17 @@ __global__ void __launch_bounds__(N / 16 * W) mfma_stockham_kernel(IO io, const 
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int j = tl_j[q];
-                const int pbase = (j / ns) * ns * 16 + (j % ns);
+                const int slot = tl_line[q] * G::pitch + mf_slot_base(j, ns);
+                // next stage's twiddle of point p: jm2 = p % ns2 = j%ns + v*ns and
+                // r2 = p / TL = (j/ns) / (TL/ns2) (ns2 divides TL; j%ns + v*ns < ns2)
+                const int twb = two + (j % ns) * 16 + (j / ns) / (TL / ns2 > 0 ? TL / ns2 : 1);
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    const int p = pbase + (4 * grp + u) * ns;
+                    const int vv = 4 * grp + u;
                     cpx<float> v{outr[q][u], outi[q][u]};
-                    if (tw_next) {
-                        const int r2 = p / TL, jm2 = (p % TL) % ns2;
-                        v = cmul(v, tw_dir<DIR>(tw[two + jm2 * 16 + r2]));
-                    }
-                    lre[tl_line[q] * G::pitch + lds_pos<N>(p)] = v.x;
-                    lim[tl_line[q] * G::pitch + lds_pos<N>(p)] = v.y;
+                    if (tw_next) v = cmul(v, tw_dir<DIR>(tw[twb + vv * ns * 16]));
+                    lre[slot + mf_slot_step(ns, vv)] = v.x;
+                    lim[slot + mf_slot_step(ns, vv)] = v.y;
                 }
             }
             __syncthreads();
@@ -169,8 +212,15 @@ __global__ void __launch_bounds__(N / 16 * W) mfma_stockham_kernel(IO io, const 
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const float* src = (part ? lim : lre) + tl_line[q] * G::pitch;
+                    if constexpr (TL % 32 == 0) {
+                        // j + (rh+u)*TL: the padding of the multiple of TL is exact
+                        const float* s0 = src + lds_pos<N>(tl_j[q]) + rh * (TL + TL / 32);
 #pragma unroll
-                    for (int u = 0; u < 8; u++) b[q][u] = src[lds_pos<N>(tl_j[q] + (rh + u) * TL)];
+                        for (int u = 0; u < 8; u++) b[q][u] = s0[u * (TL + TL / 32)];
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < 8; u++) b[q][u] = src[lds_pos<N>(tl_j[q] + (rh + u) * TL)];
+                    }
                 }
             }
         }
@@ -180,6 +230,8 @@ __global__ void __launch_bounds__(N / 16 * W) mfma_stockham_kernel(IO io, const 
     // ---- closing stage / staged output (the engine's layout)
     if constexpr (NMF < NS_ALL || COLMODE) {
         constexpr int R = RL == 16 ? 1 : RL;  // 1: no closing stage, only the store
+        IO eio = io;  // the store undoes the line's scaling
+        eio.scale = io.scale * mf_pow2(STEP - mf_line_shift(lmax[ew]));
         cpx<float> v[16];
         if constexpr (R == 1) {
 #pragma unroll
@@ -189,7 +241,7 @@ __global__ void __launch_bounds__(N / 16 * W) mfma_stockham_kernel(IO io, const 
             }
             if (eact) {
 #pragma unroll
-                for (int i = 0; i < 16; i++) io.store(ebase, et + TL * i, v[i]);
+                for (int i = 0; i < 16; i++) eio.store(ebase, et + TL * i, v[i]);
             }
         } else {
             constexpr int NSL = N / R;
@@ -201,7 +253,7 @@ __global__ void __launch_bounds__(N / 16 * W) mfma_stockham_kernel(IO io, const 
 #pragma unroll
                 for (int q = 0; q < 16 / R; q++)
 #pragma unroll
-                    for (int r = 0; r < R; r++) io.store(ebase, et + TL * (q + r * (16 / R)), v[q * R + r]);
+                    for (int r = 0; r < R; r++) eio.store(ebase, et + TL * (q + r * (16 / R)), v[q * R + r]);
             }
         }
     }

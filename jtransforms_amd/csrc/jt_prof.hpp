@@ -6,6 +6,7 @@
 
 #include <atomic>
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <string>
@@ -31,12 +32,33 @@ inline std::vector<ProfRec>& prof_list() {
     return v;
 }
 
+// JT_LAUNCH_LOG=<path>: every launch label is appended to that file in
+// launch order (one line per kernel dispatch), so that a profiler's
+// per-dispatch records can be matched to the library's launches.
+inline FILE* launch_log() {
+    static FILE* f = []() -> FILE* {
+        const char* p = std::getenv("JT_LAUNCH_LOG");
+        return (p && p[0]) ? std::fopen(p, "a") : nullptr;
+    }();
+    return f;
+}
+
 // Records an event pair around the launches made in its lifetime.
 class ProfScope {
    public:
     template <class NameFn>
     ProfScope(hipStream_t st, NameFn&& fn) : st_(st) {
+        if (FILE* lf = launch_log()) {
+            const std::string label = fn();
+            std::lock_guard<std::mutex> g(prof_mu());
+            std::fprintf(lf, "%s\n", label.c_str());
+            std::fflush(lf);
+        }
         if (!prof_on().load(std::memory_order_relaxed)) return;
+        // launches recorded into a graph under stream capture are not timed
+        // (their events would only be recorded when the graph replays)
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(st_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
         active_ = true;
         name_ = fn();
         // timing-only events: no system-scope fence (cache writeback and

@@ -291,7 +291,13 @@ __global__ void r2r_fb_post(const cpx<T>* __restrict__ y, T* __restrict__ x, Lin
     }
 }
 
-// yTransform of the multi-D DHT (DoubleDHT_2D.java:1288-1309, DoubleDHT_3D.java:2314-2356)
+// The multi-D DHT: separable 1D transforms along each axis give
+// T(k) = sum x cas(a1) cas(a2) (...); the reference then turns T into the
+// transform with cas(a1 + a2 (+ a3)) (its yTransform, DoubleDHT_2D.java:
+// 1288-1309, DoubleDHT_3D.java:2314-2356).  2D: cas(x+y) = (cas(x)cas(y) +
+// cas(-x)cas(y) + cas(x)cas(-y) - cas(-x)cas(-y)) / 2, so over the group
+// {r, -r} x {c, -c} with T values A, B, C, D at (r,c), (-r,c), (r,-c),
+// (-r,-c) each member moves by E = (A + D - B - C) / 2 (A - E, B + E, ...).
 template <typename T>
 __global__ void ytransform2d(T* __restrict__ a, int64_t rows, int64_t cols) {
     int64_t nr = rows / 2 + 1, nc = cols / 2 + 1;
@@ -312,31 +318,31 @@ __global__ void ytransform2d(T* __restrict__ a, int64_t rows, int64_t cols) {
     }
 }
 
+// 3D: from cas(x+y+z) = (cas(-x)cas(y)cas(z) + cas(x)cas(-y)cas(z) +
+// cas(x)cas(y)cas(-z) - cas(-x)cas(-y)cas(-z)) / 2 the true transform at k is
+//   H(k) = (T(-k1,k2,k3) + T(k1,-k2,k3) + T(k1,k2,-k3) - T(-k1,-k2,-k3)) / 2
+// of the separable one T.  One thread per orbit of (s, r, c) under negation
+// of each index: member b (bit 0 slice, bit 1 row, bit 2 column negated)
+// gets (T[b^1] + T[b^2] + T[b^4] - T[b^7]) / 2; self-mirrored indices give
+// coinciding members that receive equal values.
 template <typename T>
 __global__ void ytransform3d(T* __restrict__ a, int64_t sl, int64_t rows, int64_t cols) {
-    int64_t ns = sl / 2 + 1, nr = rows / 2 + 1, nc = cols / 2 + 1;
-    const int64_t ss = rows * cols, rs = cols;
+    const int64_t ns = sl / 2 + 1, nr = rows / 2 + 1, nc = cols / 2 + 1;
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     for (; i < ns * nr * nc; i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t s = i / (nr * nc), r = (i / nc) % nr, c = i % nc;
-        int64_t sC = (sl - s) % sl, rC = (rows - r) % rows, cC = (cols - c) % cols;
-        int64_t i1 = s * ss + rC * rs + c, i2 = s * ss + r * rs + cC;
-        int64_t i3 = sC * ss + r * rs + c, i4 = sC * ss + rC * rs + cC;
-        int64_t i5 = sC * ss + rC * rs + c, i6 = sC * ss + r * rs + cC;
-        int64_t i7 = s * ss + r * rs + c, i8 = s * ss + rC * rs + cC;
-        T A = a[i1], B = a[i2], C = a[i3], D = a[i4];
-        T E = a[i5], F = a[i6], G = a[i7], H = a[i8];
-        T n7 = (A + B + C - D) / 2, n3 = (E + F + G - H) / 2, n1 = (G + H + E - F) / 2;
-        T n5 = (C + D + A - B) / 2, n2 = (H + G + F - E) / 2, n6 = (D + C + B - A) / 2;
-        T n8 = (B + A + D - C) / 2, n4 = (F + E + H - G) / 2;
-        a[i7] = n7;
-        a[i3] = n3;
-        a[i1] = n1;
-        a[i5] = n5;
-        a[i2] = n2;
-        a[i6] = n6;
-        a[i8] = n8;
-        a[i4] = n4;
+        const int64_t s = i / (nr * nc), r = (i / nc) % nr, c = i % nc;
+        const int64_t sp[2] = {s * rows * cols, ((sl - s) % sl) * rows * cols};
+        const int64_t rp[2] = {r * cols, ((rows - r) % rows) * cols};
+        const int64_t cp[2] = {c, (cols - c) % cols};
+        int64_t at[8];
+        T t[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            at[m] = sp[m & 1] + rp[(m >> 1) & 1] + cp[(m >> 2) & 1];
+            t[m] = a[at[m]];
+        }
+#pragma unroll
+        for (int m = 0; m < 8; m++) a[at[m]] = (t[m ^ 1] + t[m ^ 2] + t[m ^ 4] - t[m ^ 7]) / 2;
     }
 }
 

@@ -10,7 +10,7 @@ namespace jt {
 
 // packed post pass in place over lines of N complex (= 2N reals), pairs (k, N-k)
 template <typename T>
-__global__ void r2c_post_inplace(cpx<T>* __restrict__ z, const cpx<T>* __restrict__ rtw, int64_t N,
+__global__ void r2c_post_inplace(cpx<T>* __restrict__ z, const cpx<T>* __restrict__ rtw, int64_t loff, int64_t N,
                                  int64_t lines, T scale) {
     int64_t half = N / 2 + 1;
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -28,7 +28,7 @@ __global__ void r2c_post_inplace(cpx<T>* __restrict__ z, const cpx<T>* __restric
         cpx<T> B = {Zc.x, -Zc.y};
         cpx<T> fe = cadd(A, B), d = csub(A, B);
         cpx<T> fo = {d.y, -d.x};
-        cpx<T> wf = cmul(rtw[k], fo);
+        cpx<T> wf = cmul(cmul(rtw[k >> 8], rtw[loff + (k & 255)]), fo);
         T h = T(0.5) * scale;
         a[k] = cpx<T>{(fe.x + wf.x) * h, (fe.y + wf.y) * h};
         if (kk != k) a[kk] = cpx<T>{(fe.x - wf.x) * h, -(fe.y - wf.y) * h};
@@ -37,7 +37,7 @@ __global__ void r2c_post_inplace(cpx<T>* __restrict__ z, const cpx<T>* __restric
 
 // packed pre pass in place (inverse of r2c_post up to the factor 2N)
 template <typename T>
-__global__ void c2r_pre_inplace(cpx<T>* __restrict__ z, const cpx<T>* __restrict__ rtw, int64_t N,
+__global__ void c2r_pre_inplace(cpx<T>* __restrict__ z, const cpx<T>* __restrict__ rtw, int64_t loff, int64_t N,
                                 int64_t lines) {
     int64_t half = N / 2 + 1;
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -54,7 +54,8 @@ __global__ void c2r_pre_inplace(cpx<T>* __restrict__ z, const cpx<T>* __restrict
         cpx<T> Xn = a[kk];
         cpx<T> Xc = {Xn.x, -Xn.y};
         cpx<T> E = cadd(Xk, Xc), D = csub(Xk, Xc);
-        cpx<T> w = {rtw[k].x, -rtw[k].y};
+        const cpx<T> wk = cmul(rtw[k >> 8], rtw[loff + (k & 255)]);
+        cpx<T> w = {wk.x, -wk.y};
         cpx<T> O = cmul(D, w);
         a[k] = cpx<T>{E.x - O.y, E.y + O.x};
         if (kk != k) a[kk] = cpx<T>{E.x + O.y, O.x - E.y};
@@ -132,7 +133,12 @@ struct Real1D {
     int64_t n = 0;
     std::unique_ptr<Fft1D<T>> full;  // length n (odd n, *Full ops)
     std::unique_ptr<Fft1D<T>> half;  // length n/2 (even n)
-    DevBuf rtw;                      // exp(-2 pi i k/n), k in [0, n/4]
+    // w^k = exp(-2 pi i k/n): with a one-pass half plan, w^k for k in
+    // [0, n/4 + 1] (the IO hooks of its kernel); otherwise (large n) as two levels,
+    // w^k = rtw[k >> 8] * rtw[rlo + (k & 255)] for k in [0, n/4 + 1], so the
+    // table stays O(n / 256) long
+    DevBuf rtw;
+    int64_t rlo = 0;
     int64_t ws_elems(int64_t lines) const {
         int64_t w = 0;
         if (half) w = std::max<int64_t>(w, half->workspace(lines));
@@ -148,8 +154,16 @@ std::unique_ptr<Real1D<T>> make_real1d(int64_t n) {
     r->full = make_fft1d<T>(n);
     if (n % 2 == 0 && n >= 2) {
         r->half = make_fft1d<T>(n / 2);
-        std::vector<cpx<T>> h(n / 4 + 2);
-        for (int64_t k = 0; k < (int64_t)h.size(); k++) h[k] = expm<T>(k, n);
+        std::vector<cpx<T>> h;
+        if (r->half->single()) {
+            h.resize(n / 4 + 2);
+            for (int64_t k = 0; k < (int64_t)h.size(); k++) h[k] = expm<T>(k, n);
+        } else {
+            const int64_t nhi = ((n / 4 + 1) >> 8) + 1;
+            for (int64_t i = 0; i < nhi; i++) h.push_back(expm<T>(i << 8, n));
+            r->rlo = nhi;
+            for (int64_t i = 0; i < 256; i++) h.push_back(expm<T>(i, n));
+        }
         upload(r->rtw, h);
     }
     return r;
@@ -171,7 +185,7 @@ void real_forward(const Real1D<T>& r, T* a, int64_t lines, cpx<T>* ws, hipStream
             c2c_contig<T, -1>(*r.half, z, z, lines, T(1), ws, st);
             { JT_PROF(st, "r2c_post_inplace");
             hipLaunchKernelGGL((r2c_post_inplace<T>), ew_grid(lines * (N / 2 + 1)), dim3(256), 0, st, z,
-                               r.rtw.template as<cpx<T>>(), N, lines, T(1));
+                               r.rtw.template as<cpx<T>>(), r.rlo, N, lines, T(1));
             }
             JT_HIP(hipGetLastError());
         }
@@ -214,7 +228,7 @@ void real_inverse(const Real1D<T>& r, T* a, int64_t lines, bool scale, cpx<T>* w
         } else {
             { JT_PROF(st, "c2r_pre_inplace");
             hipLaunchKernelGGL((c2r_pre_inplace<T>), ew_grid(lines * (N / 2 + 1)), dim3(256), 0, st, z,
-                               r.rtw.template as<cpx<T>>(), N, lines);
+                               r.rtw.template as<cpx<T>>(), r.rlo, N, lines);
             }
             JT_HIP(hipGetLastError());
             c2c_contig<T, 1>(*r.half, z, z, lines, f, ws, st);
