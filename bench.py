@@ -84,6 +84,17 @@ def alg_bytes(name):
     return 2.0 * lines * n * 2 * es
 
 
+def ragged_alg_bytes(prec, lens):
+    """Algorithmic HBM bytes of one ragged-batch launch: every point read and
+    written once per pass, one pass up to the single-pass limit (4096 points
+    FP64, 8192 FP32), two (four-step) above; Bluestein lengths counted at n."""
+    import numpy as np
+    es = 16 if prec == "f64" else 8
+    single = 4096 if prec == "f64" else 8192
+    passes = np.where(lens <= single, 1, 2)
+    return float(np.sum(2.0 * es * lens * passes))
+
+
 def load_json(path):
     try:
         with open(path) as f:
@@ -273,12 +284,7 @@ def other_workload(args, torch, jt):
         res = {}
         for prec in ("f64", "f32"):
             dt = torch.float64 if prec == "f64" else torch.float32
-            es = 16 if prec == "f64" else 8
-            # algorithmic bytes: read + write once per pass; 2 passes above
-            # the single-pass limit (4096 points FP64, 8192 FP32)
-            single = 4096 if prec == "f64" else 8192
-            passes = np.where(lens <= single, 1, 2)
-            byt = float(np.sum(2.0 * es * lens * passes))
+            byt = ragged_alg_bytes(prec, lens)
             for graph in ((False,) if args.no_graph else (False, True)):
                 print(f"ragged {prec} graph={graph}", file=sys.stderr, flush=True)
                 t0 = time.perf_counter()
@@ -394,6 +400,10 @@ def summarize_pmc(pmc_dir):
             wr = sum(v for v, _ in wf[key]) / len(wf[key])
             al = [a for _, a in ff[key] if a]
             alg = sum(al) / len(al) if al else None
+            m = re.match(r"ragged<(f64|f32)>", key)
+            if m and w == "ragged":
+                from jtransforms_amd.workloads import ragged_lengths
+                alg = ragged_alg_bytes(m.group(1), ragged_lengths(20000))
             tot = fe + wr
             data["kernels"][key] = {"hbm_bytes_per_launch": tot, "fetch_bytes": fe, "write_bytes": wr,
                                     "alg_bytes_per_launch": alg, "launches": len(ff[key]),
