@@ -6,12 +6,16 @@
 // Here every length gets a per-length descriptor (radix plan, four-step split
 // or Bluestein convolution length) and the whole set becomes a queue of work
 // items executed by a persistent kernel:
-//   SINGLE     a length that fits one workgroup's LDS
+//   SINGLE     transforms of one length that fits one workgroup's LDS
 //   FOUR_A/B   the column / row halves of a four-step length (n > LDS limit)
-//   BLUE_S     Bluestein with its convolution length m in LDS (all in one item)
+//   BLUE_S     Bluestein transforms of one length whose convolution length m
+//              fits LDS (premul, FFT_m, spectrum product, inverse FFT_m,
+//              postmul in one residency)
 //   BLUE_1..3  Bluestein with a four-step m: premul+columns; rows (forward
 //              FFT, spectrum product, inverse FFT, twiddle in one residency);
 //              inverse columns+postmul
+// Every item runs the same round loop (lines into LDS, the stages, lines
+// out), so the kernel holds one copy of the stage code.
 // Workgroups take items in queue order with an atomic counter.  An item that
 // consumes another phase's output waits on a per-transform counter; every
 // producer of that phase sits earlier in the queue and was therefore taken by
@@ -48,8 +52,10 @@ struct RLen {
     int tpl;                    // fewest threads per line for the in-place stages
 };
 
-// Loads in flight per thread when a round's lines are read into LDS.
+// Loads in flight per thread when a round's lines are read into LDS, and
+// points per thread per step of the stores.
 constexpr int RLB = 8;
+constexpr int RSB = 4;
 
 // In-place stages hold at most 16 points per thread.
 template <typename T>
@@ -71,7 +77,7 @@ struct RArgs {
     const RLen* lens;
     const cpx<T>* tw;
     const RItem* items;
-    const int64_t* tlist;  // transform indices of the grouped single items
+    const int64_t* tlist;  // transform indices of the grouped whole-transform items
     int nitems;
     int* next;     // queue head
     int* cnt;      // 3 phase counters per transform
@@ -108,13 +114,28 @@ struct RTune {
 // device helpers
 // ---------------------------------------------------------------------------
 
+// The index of the queue item, read from LDS, is the same in every lane; the
+// compiler cannot tell, so it is moved to a scalar register explicitly.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Read-only tables of the plan (descriptors, items, offsets, twiddles) are
+// read through the constant address space: the compiler then knows that the
+// kernel's stores never change them, so their loads are hoisted and batched
+// across stores, and loads at a wave-uniform address become scalar loads.
+template <class X>
+using ro_ptr = const __attribute__((address_space(4))) X*;
+template <class X>
+__device__ __forceinline__ ro_ptr<X> ro(const X* p) { return (ro_ptr<X>)p; }
+template <typename T>
+__device__ __forceinline__ cpx<T> rd(ro_ptr<cpx<T>> p, int64_t i) { return cpx<T>{p[i].x, p[i].y}; }
+
 // One Stockham stage of radix R (span ns) in place on a line's LDS arrays:
 // thread tl of the line's tpl threads gathers the inputs of its butterflies
 // j = tl + b*tpl (b < PM/R; the host sizes tpl so that this covers all n/R)
 // into registers, the workgroup synchronises, and the outputs overwrite the
 // same arrays.  Every thread of the workgroup calls this (the barrier).
 template <int R, int DIR, int PM, typename T>
-__device__ __forceinline__ void stage_ip(T* re, T* im, const cpx<T>* __restrict__ tws, int n, int ns, int tl,
+__device__ __forceinline__ void stage_ip(T* re, T* im, ro_ptr<cpx<T>> tws, int n, int ns, int tl,
                                          int tpl, bool act) {
     constexpr int NB = PM / R;
     const int m = n / R;
@@ -140,7 +161,7 @@ __device__ __forceinline__ void stage_ip(T* re, T* im, const cpx<T>* __restrict_
             const int jm = j - jh * ns;
             if (ns > 1) {
 #pragma unroll
-                for (int r = 1; r < R; r++) v[b][r] = cmul(v[b][r], tw_dir<DIR>(tws[jm * (R - 1) + r - 1]));
+                for (int r = 1; r < R; r++) v[b][r] = cmul(v[b][r], tw_dir<DIR>(rd(tws, jm * (R - 1) + r - 1)));
             }
             dft_any<R, DIR>(v[b]);
             const int d = jh * ns * R + jm;
@@ -155,7 +176,7 @@ __device__ __forceinline__ void stage_ip(T* re, T* im, const cpx<T>* __restrict_
 }
 
 template <int DIR, int PM, typename T>
-__device__ __forceinline__ void stage_ip_rt(int R, T* re, T* im, const cpx<T>* __restrict__ tws, int n, int ns,
+__device__ __forceinline__ void stage_ip_rt(int R, T* re, T* im, ro_ptr<cpx<T>> tws, int n, int ns,
                                             int tl, int tpl, bool act) {
     switch (R) {
         case 2: stage_ip<2, DIR, PM>(re, im, tws, n, ns, tl, tpl, act); break;
@@ -170,132 +191,128 @@ __device__ __forceinline__ void stage_ip_rt(int R, T* re, T* im, const cpx<T>* _
     }
 }
 
-// All stages of L in place; the workgroup's threads all call this.
+// All stages of L in place, forward direction; the workgroup's threads all
+// call this.  An inverse transform runs the same stages on conjugated data
+// (the inverse DFT of x is conj(DFT(conj x))): the callers conjugate on the
+// way into LDS and on the way out, so the kernel holds one copy of the stage
+// code for both directions.
 template <typename T>
-__device__ __forceinline__ void stages_ip(int dir, const RLen& L, const cpx<T>* __restrict__ tw, T* re, T* im,
-                                          int tl, int tpl, bool act) {
+__device__ __forceinline__ void stages_fwd(ro_ptr<RLen> L, ro_ptr<cpx<T>> tw, T* re, T* im, int tl,
+                                           int tpl, bool act) {
     constexpr int PM = ragged_pts<T>();
+    const int nst = L->nst, n = L->n;
     int ns = 1;
-    for (int s = 0; s < L.nst; s++) {
+    for (int s = 0; s < nst; s++) {
         __syncthreads();
-        const int R = L.radix[s];
-        if (dir < 0) stage_ip_rt<-1, PM>(R, re, im, tw + L.twoff[s], L.n, ns, tl, tpl, act);
-        else stage_ip_rt<1, PM>(R, re, im, tw + L.twoff[s], L.n, ns, tl, tpl, act);
+        const int R = L->radix[s];
+        // an empty asm that "changes" the lane index: the LDS addresses of a
+        // stage are then computed inside the stage, not hoisted out of the
+        // loops and held in registers for every stage and round (which took
+        // the kernel to 256 VGPRs plus scratch; with it, 127 and none)
+        int tls = tl;
+        asm volatile("" : "+v"(tls));
+        stage_ip_rt<-1, PM>(R, re, im, tw + L->twoff[s], n, ns, tls, tpl, act);
         ns *= R;
     }
     __syncthreads();
 }
 
-// Transform `cnt` lines of length L.n held in LDS, G lines at a time: the
-// workgroup's threads split into G groups of tpl threads, each group owning
-// one line's slice of the two LDS arrays (re, im; each np_total reals).
-// load(line, k) / store(line, k, v) move point k of line `line` in [0, cnt).
-// All threads take every barrier.
-// TWO: after the first transform (direction dir) mid(line, k, v) maps every
-// point in LDS and a second transform (direction dir2) follows before the
-// store (Bluestein rows: FFT, spectrum product, inverse FFT in one residency).
-template <typename T, bool LOAD_LF, bool STORE_LF, bool TWO, class LoadF, class MidF, class StoreF>
-__device__ __forceinline__ void rlines_impl(int dir, int dir2, int tplx, const RLen& L,
-                                            const cpx<T>* __restrict__ tw, T* sm, int np_total, int cnt,
-                                            LoadF load, MidF mid, StoreF store) {
-    const int n = L.n;
-    const int npl = n + (n >> 5) + 1;
-    const int NT = blockDim.x;
-    int tpl = min(NT, L.tpl << tplx);
-    int G = NT / tpl;
-    const int glds = np_total / npl;
-    if (G > glds) G = glds;
-    if (G < 1) G = 1;
-    // when LDS caps the line count, the spare threads widen each line
-    while (2 * tpl * G <= NT && tpl < n) tpl <<= 1;
-    const int t = threadIdx.x;
-    const int g = t / tpl, tl = t % tpl;
-    T* are = sm + g * npl;
-    T* aim = sm + np_total + g * npl;
-    for (int base = 0; base < cnt; base += G) {
-        const int line = base + g;
-        const bool act = g < G && line < cnt;
-        __syncthreads();
-        if constexpr (LOAD_LF) {
-            // lanes run across the G lines (adjacent columns in memory);
-            // RLB loads in flight per thread before the LDS writes
-            const int nl = min(G, cnt - base);
-            const int tot = nl * n;
-            const float rnl = 1.0f / (float)nl;
-            for (int i0 = t; i0 < tot; i0 += RLB * NT) {
-                cpx<T> v[RLB];
+// Loading lines into LDS.  Line `lg` of the round (lg < nl) occupies
+// sm[lg*npl + gpos(k)] (real parts) and sm[np + lg*npl + gpos(k)] (imaginary
+// parts).  cj conjugates on the way in.
+// Lanes across lines: consecutive threads take consecutive lines of one
+// point (adjacent columns in memory); RLB loads in flight per thread.
+template <typename T, class F>
+__device__ __forceinline__ void rload_across(T* sm, int np, int npl, int n, int nl, int base, bool cj, F load) {
+    const int t = threadIdx.x, NT = blockDim.x;
+    const int tot = nl * n;
+    const float rnl = 1.0f / (float)nl;
+    for (int i0 = t; i0 < tot; i0 += RLB * NT) {
+        cpx<T> v[RLB];
 #pragma unroll
-                for (int u = 0; u < RLB; u++) {
-                    const int idx = i0 + u * NT;
-                    if (idx < tot) {
-                        const int k = fdiv(idx, nl, rnl);
-                        v[u] = load(base + idx - k * nl, k);
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < RLB; u++) {
-                    const int idx = i0 + u * NT;
-                    if (idx < tot) {
-                        const int k = fdiv(idx, nl, rnl), lg = idx - k * nl;
-                        sm[lg * npl + gpos(k)] = v[u].x;
-                        sm[np_total + lg * npl + gpos(k)] = v[u].y;
-                    }
-                }
-            }
-        } else if (act) {
-            for (int k0 = tl; k0 < n; k0 += RLB * tpl) {
-                cpx<T> v[RLB];
-#pragma unroll
-                for (int u = 0; u < RLB; u++) {
-                    const int k = k0 + u * tpl;
-                    if (k < n) v[u] = load(line, k);
-                }
-#pragma unroll
-                for (int u = 0; u < RLB; u++) {
-                    const int k = k0 + u * tpl;
-                    if (k < n) {
-                        are[gpos(k)] = v[u].x;
-                        aim[gpos(k)] = v[u].y;
-                    }
-                }
+        for (int u = 0; u < RLB; u++) {
+            const int idx = i0 + u * NT;
+            if (idx < tot) {
+                const int k = fdiv(idx, nl, rnl);
+                v[u] = load(base + idx - k * nl, k);
             }
         }
-        stages_ip<T>(dir, L, tw, are, aim, tl, tpl, act);
-        if constexpr (TWO) {
-            if (act) {
-                for (int k = tl; k < n; k += tpl) {
-                    cpx<T> v = mid(line, k, cpx<T>{are[gpos(k)], aim[gpos(k)]});
-                    are[gpos(k)] = v.x;
-                    aim[gpos(k)] = v.y;
-                }
-            }
-            stages_ip<T>(dir2, L, tw, are, aim, tl, tpl, act);
-        }
-        if constexpr (STORE_LF) {
-            const int nl = min(G, cnt - base);
-            const float rnl = 1.0f / (float)nl;
-            for (int idx = t; idx < nl * n; idx += NT) {
+#pragma unroll
+        for (int u = 0; u < RLB; u++) {
+            const int idx = i0 + u * NT;
+            if (idx < tot) {
                 const int k = fdiv(idx, nl, rnl), lg = idx - k * nl;
-                store(base + lg, k, cpx<T>{sm[lg * npl + gpos(k)], sm[np_total + lg * npl + gpos(k)]});
+                sm[lg * npl + gpos(k)] = v[u].x;
+                sm[np + lg * npl + gpos(k)] = cj ? -v[u].y : v[u].y;
             }
-        } else if (act) {
-            for (int k = tl; k < n; k += tpl) store(line, k, cpx<T>{are[gpos(k)], aim[gpos(k)]});
         }
     }
 }
 
-template <typename T, bool LOAD_LF, bool STORE_LF, class LoadF, class StoreF>
-__device__ __forceinline__ void rlines(int dir, int tplx, const RLen& L, const cpx<T>* __restrict__ tw, T* sm,
-                                       int np_total, int cnt, LoadF load, StoreF store) {
-    rlines_impl<T, LOAD_LF, STORE_LF, false>(dir, dir, tplx, L, tw, sm, np_total, cnt, load,
-                                             [](int, int, cpx<T> v) { return v; }, store);
+// Lanes along a line: the line's tpl threads take consecutive points.
+template <typename T, class F>
+__device__ __forceinline__ void rload_along(T* are, T* aim, int n, int line, int tl, int tpl, bool act, bool cj,
+                                            F load) {
+    if (!act) return;
+    for (int k0 = tl; k0 < n; k0 += RLB * tpl) {
+        cpx<T> v[RLB];
+#pragma unroll
+        for (int u = 0; u < RLB; u++) {
+            const int k = k0 + u * tpl;
+            if (k < n) v[u] = load(line, k);
+        }
+#pragma unroll
+        for (int u = 0; u < RLB; u++) {
+            const int k = k0 + u * tpl;
+            if (k < n) {
+                are[gpos(k)] = v[u].x;
+                aim[gpos(k)] = cj ? -v[u].y : v[u].y;
+            }
+        }
+    }
+}
+
+// Stores, RSB per thread at a time: the table loads of a store epilogue
+// (twiddles, chirp) of the RSB points are then issued together.
+template <typename T, class F>
+__device__ __forceinline__ void rstore_across(const T* sm, int np, int npl, int n, int nl, int base, bool cj,
+                                              F store) {
+    const int t = threadIdx.x, NT = blockDim.x;
+    const int tot = nl * n;
+    const float rnl = 1.0f / (float)nl;
+    for (int i0 = t; i0 < tot; i0 += RSB * NT) {
+#pragma unroll
+        for (int u = 0; u < RSB; u++) {
+            const int idx = i0 + u * NT;
+            if (idx < tot) {
+                const int k = fdiv(idx, nl, rnl), lg = idx - k * nl;
+                const T y = sm[np + lg * npl + gpos(k)];
+                store(base + lg, k, cpx<T>{sm[lg * npl + gpos(k)], cj ? -y : y});
+            }
+        }
+    }
+}
+
+template <typename T, class F>
+__device__ __forceinline__ void rstore_along(const T* are, const T* aim, int n, int line, int tl, int tpl, bool act,
+                                             bool cj, F store) {
+    if (!act) return;
+    for (int k0 = tl; k0 < n; k0 += RSB * tpl) {
+#pragma unroll
+        for (int u = 0; u < RSB; u++) {
+            const int k = k0 + u * tpl;
+            if (k < n) {
+                const T y = aim[gpos(k)];
+                store(line, k, cpx<T>{are[gpos(k)], cj ? -y : y});
+            }
+        }
+    }
 }
 
 template <typename T>
-__device__ __forceinline__ cpx<T> rtw2(int dir, const cpx<T>* __restrict__ tw, const RLen& L, int64_t e,
-                                       int64_t N) {
-    (void)N;  // e = j2*k1 < n2*n1 = N: no reduction needed
-    cpx<T> w = cmul(tw[L.hioff + (e >> 8)], tw[L.looff + (e & 255)]);
+__device__ __forceinline__ cpx<T> rtw2(int dir, ro_ptr<cpx<T>> tw, ro_ptr<RLen> L, int64_t e) {
+    // e = j2*k1 < n2*n1: no reduction needed
+    cpx<T> w = cmul(rd(tw, L->hioff + (e >> 8)), rd(tw, L->looff + (e & 255)));
     return dir < 0 ? w : cpx<T>{w.x, -w.y};
 }
 
@@ -350,165 +367,173 @@ template <typename T, int NTMAX>
 __global__ void __launch_bounds__(NTMAX) ragged_kernel(RArgs<T> a, int np) {
     extern __shared__ unsigned char smem_raw[];
     T* sm = reinterpret_cast<T*>(smem_raw);
-    T* are = sm;
-    T* aim = sm + np;
     __shared__ int cur;
-    const int t = threadIdx.x, nt = blockDim.x;
+    const int t = threadIdx.x, NT = blockDim.x;
     const int dir = a.dir;
     for (;;) {
         __syncthreads();
         if (t == 0) cur = atomicAdd(a.next, 1);
         __syncthreads();
-        const int it = cur;
+        const int it = uni(cur);
         if (it >= a.nitems) break;  // uniform across the workgroup
-        const RItem item = a.items[it];
-        const RLen& L = a.lens[item.len];
-        const int64_t tr = item.tr;
-        const T scale = a.scaled ? T(1) / T(L.n) : T(1);
-        cpx<T>* x = a.data + a.off[tr];
+        ro_ptr<RItem> item = ro(a.items) + it;
+        const int type = item->type;
+        const int64_t tr = item->tr;
+        const int c0 = item->c0, cnt = item->cnt;
+        ro_ptr<RLen> lens = ro(a.lens);
+        ro_ptr<RLen> L = lens + item->len;
+        // B: the four-step length (of the transform, or of its Bluestein
+        // convolution); for RT_BLUE_S the in-LDS convolution length
+        ro_ptr<RLen> B = type >= RT_BLUE_S ? lens + L->lm : L;
+        const int n1 = B->n1, n2 = B->n2;
         int* c3 = a.cnt + 3 * tr;
-        switch (item.type) {
-            case RT_SINGLE: {
-                // cnt transforms of this length, listed from tlist[c0]
-                const int64_t* tl0 = a.tlist + item.c0;
-                const cpx<T>* data = a.data;
-                const int64_t* off = a.off;
-                rlines<T, false, false>(dir, a.tplx, L, a.tw, sm, np, item.cnt,
-                          [=](int line, int k) { return data[off[tl0[line]] + k]; },
-                          [=](int line, int k, cpx<T> v) {
-                              a.data[off[tl0[line]] + k] = cpx<T>{v.x * scale, v.y * scale};
-                          });
-                break;
-            }
-            case RT_FOUR_A:
-            case RT_BLUE_1: {
-                // columns j2 in [c0, c0+cnt) of the n1 x n2 view; FFT_n1 then
-                // the exp(d 2 pi i j2 k1 / N) twiddle, into the workspace
-                // (Bluestein: of the chirp-premultiplied, zero-padded input)
-                const bool blue = item.type == RT_BLUE_1;
-                const RLen& B = blue ? a.lens[L.lm] : L;  // the four-step length
-                const RLen& L1 = a.lens[B.l1];
-                const int n2 = B.n2;
-                const int64_t N = (int64_t)B.n1 * n2;
-                const int d = blue ? -1 : dir;
-                cpx<T>* wsA = a.ws + a.wsoff[tr];
-                const cpx<T>* src = x;
-                const int c0 = item.c0, nreal = L.n;
-                const cpx<T>* chirp = a.tw + L.choff;
-                rlines<T, true, true>(d, a.tplx, L1, a.tw, sm, np, item.cnt,
-                          [=](int line, int j1) {
-                              const int e = j1 * n2 + (c0 + line);
-                              if (blue) {
-                                  if (e >= nreal) return cpx<T>{T(0), T(0)};
-                                  return dir < 0 ? cmulc(src[e], chirp[e]) : cmul(src[e], chirp[e]);
-                              }
-                              return src[e];
-                          },
-                          [=](int line, int k1, cpx<T> v) {
-                              int j2 = c0 + line;
-                              wsA[k1 * n2 + j2] = cmul(v, rtw2<T>(d, a.tw, B, j2 * k1, N));
-                          });
-                rsignal(c3 + 0, item.cnt);
-                break;
-            }
-            case RT_FOUR_B: {
-                // rows k1 in [c0, c0+cnt): FFT_n2 of the workspace row; output
-                // point k2 is element k1 + n1*k2
-                const RLen& L2 = a.lens[L.l2];
-                const int n1 = L.n1, n2 = L.n2;
-                cpx<T>* wsA = a.ws + a.wsoff[tr];
-                if (!rwait(c3 + 0, n2 + a.wait_extra, a.err, a.wait_ticks)) break;
-                const int c0 = item.c0;
-                rlines<T, false, true>(dir, a.tplx, L2, a.tw, sm, np, item.cnt,
-                          [=](int line, int j2) { return wsA[(c0 + line) * n2 + j2]; },
-                          [=](int line, int k2, cpx<T> v) {
-                              x[(c0 + line) + n1 * k2] = cpx<T>{v.x * scale, v.y * scale};
-                          });
-                break;
-            }
-            case RT_BLUE_2: {
-                // Bluestein rows k1 in [c0, c0+cnt) of the convolution length
-                // N = n1*n2, in one LDS residency: forward FFT_n2 (point k2 is
-                // X[k1 + n1*k2]), times the kernel spectrum, inverse FFT_n2 back
-                // to j2, times exp(+2 pi i j2 k1 / N); written back in place.
-                // (The inverse FFT_N then finishes with FFT_n1 over k1.)
-                const RLen& B = a.lens[L.lm];
-                const RLen& L2 = a.lens[B.l2];
-                const int n1 = B.n1, n2 = B.n2;
-                const int64_t N = (int64_t)n1 * n2;
-                cpx<T>* wsA = a.ws + a.wsoff[tr];
-                if (!rwait(c3 + 0, n2 + a.wait_extra, a.err, a.wait_ticks)) break;
-                const int c0 = item.c0;
-                const cpx<T>* bspec = a.tw + L.bsoff;
-                rlines_impl<T, false, false, true>(-1, 1, a.tplx, L2, a.tw, sm, np, item.cnt,
-                          [=](int line, int j2) { return wsA[(c0 + line) * n2 + j2]; },
-                          [=](int line, int k2, cpx<T> v) {
-                              const cpx<T> bb = bspec[(c0 + line) + n1 * k2];
-                              return dir < 0 ? cmul(v, bb) : cmulc(v, bb);
-                          },
-                          [=](int line, int j2, cpx<T> v) {
-                              const int k1 = c0 + line;
-                              wsA[k1 * n2 + j2] = cmul(v, rtw2<T>(1, a.tw, B, j2 * k1, N));
-                          });
-                rsignal(c3 + 1, item.cnt);
-                break;
-            }
-            case RT_BLUE_3: {
-                // Bluestein columns j2 in [c0, c0+cnt): inverse FFT_n1 over k1
-                // gives point j1*n2 + j2 of the convolution; chirp postmul of
-                // the first n points into the transform
-                const RLen& B = a.lens[L.lm];
-                const RLen& L1 = a.lens[B.l1];
-                const int n1 = B.n1, n2 = B.n2;
-                cpx<T>* wsA = a.ws + a.wsoff[tr];
-                if (!rwait(c3 + 1, n1 + a.wait_extra, a.err, a.wait_ticks)) break;
-                const int c0 = item.c0, nreal = L.n;
-                const cpx<T>* chirp = a.tw + L.choff;
-                rlines<T, true, true>(1, a.tplx, L1, a.tw, sm, np, item.cnt,
-                          [=](int line, int k1) { return wsA[k1 * n2 + (c0 + line)]; },
-                          [=](int line, int j1, cpx<T> v) {
-                              const int K = j1 * n2 + (c0 + line);
-                              if (K < nreal) {
-                                  const cpx<T> c = chirp[K];
-                                  v = dir < 0 ? cmulc(v, c) : cmul(v, c);
-                                  x[K] = cpx<T>{v.x * scale, v.y * scale};
-                              }
-                          });
-                break;
-            }
-            case RT_BLUE_S: {
-                // premul, FFT_m, spectrum product, inverse FFT_m, postmul in LDS
-                const RLen& M = a.lens[L.lm];
-                const int n = L.n, m = M.n;
-                for (int k = t; k < m; k += nt) {
-                    cpx<T> v = {T(0), T(0)};
-                    if (k < n) {
-                        cpx<T> c = a.tw[L.choff + k];
-                        v = dir < 0 ? cmulc(x[k], c) : cmul(x[k], c);
-                    }
-                    are[gpos(k)] = v.x;
-                    aim[gpos(k)] = v.y;
-                }
-                stages_ip<T>(-1, M, a.tw, are, aim, t, nt, true);
-                for (int k = t; k < m; k += nt) {
-                    cpx<T> v = {are[gpos(k)], aim[gpos(k)]};
-                    cpx<T> b = a.tw[L.bsoff + k];
-                    v = dir < 0 ? cmul(v, b) : cmulc(v, b);
-                    are[gpos(k)] = v.x;
-                    aim[gpos(k)] = v.y;
-                }
-                stages_ip<T>(1, M, a.tw, are, aim, t, nt, true);
-                for (int k = t; k < n; k += nt) {
-                    cpx<T> v = {are[gpos(k)], aim[gpos(k)]};
-                    cpx<T> c = a.tw[L.choff + k];
-                    v = dir < 0 ? cmulc(v, c) : cmul(v, c);
-                    x[k] = cpx<T>{v.x * scale, v.y * scale};
-                }
-                break;
-            }
-            default:
-                break;
+        // an item that consumes another phase's output waits for all of it
+        if (type == RT_FOUR_B || type == RT_BLUE_2) {
+            if (!rwait(c3 + 0, n2 + a.wait_extra, a.err, a.wait_ticks)) continue;
+        } else if (type == RT_BLUE_3) {
+            if (!rwait(c3 + 1, n1 + a.wait_extra, a.err, a.wait_ticks)) continue;
         }
+        // the lines this item transforms: SINGLE / BLUE_S whole transforms
+        // (listed from tlist[c0]); four-step items columns (FFT_n1) or rows
+        // (FFT_n2) c0 .. c0+cnt-1 of the n1 x n2 view
+        ro_ptr<RLen> LL = type == RT_SINGLE                         ? L
+                          : type == RT_BLUE_S                       ? B
+                          : (type == RT_FOUR_B || type == RT_BLUE_2) ? lens + B->l2
+                                                                    : lens + B->l1;
+        // Bluestein's convolution: forward pass, spectrum product, inverse pass
+        const int npass = (type == RT_BLUE_2 || type == RT_BLUE_S) ? 2 : 1;
+        const int d1 = (type == RT_BLUE_1 || npass == 2) ? -1 : type == RT_BLUE_3 ? 1 : dir;
+        const bool cj_in = d1 > 0, cj_out = (npass == 2) || d1 > 0;
+        const int nreal = L->n;
+        const T scale = a.scaled ? T(1) / T(nreal) : T(1);
+        ro_ptr<cpx<T>> tw = ro(a.tw);
+        ro_ptr<int64_t> off = ro(a.off);
+        ro_ptr<int64_t> tl0 = ro(a.tlist) + c0;
+        cpx<T>* x = a.data + off[tr];
+        cpx<T>* wsA = a.ws + ro(a.wsoff)[tr];
+        ro_ptr<cpx<T>> chirp = tw + L->choff;
+        ro_ptr<cpx<T>> bspec = tw + L->bsoff;
+        cpx<T>* data = a.data;
+
+        // G lines at a time, each owned by tpl threads and a slice of the two
+        // LDS arrays (re, im; np reals each); when LDS or the item caps the
+        // line count, the spare threads widen each line
+        const int n = LL->n;
+        const int npl = n + (n >> 5) + 1;
+        int tpl = min(NT, LL->tpl << a.tplx);
+        int G = min(min(NT / tpl, np / npl), cnt);
+        if (G < 1) G = 1;
+        while (2 * tpl * G <= NT && tpl < n) tpl <<= 1;
+        const int g = t / tpl, tl_ = t % tpl;
+        T* are = sm + g * npl;
+        T* aim = sm + np + g * npl;
+        for (int base = 0; base < cnt; base += G) {
+            int tl = tl_;
+            asm volatile("" : "+v"(tl));  // as in stages_fwd: no addresses held across rounds
+            const int line = base + g;
+            const bool act = g < G && line < cnt;
+            const int nl = min(G, cnt - base);
+            __syncthreads();
+            switch (type) {
+                case RT_SINGLE:
+                    rload_along<T>(are, aim, n, line, tl, tpl, act, cj_in,
+                                   [=](int ln, int k) { return data[off[tl0[ln]] + k]; });
+                    break;
+                case RT_FOUR_A:
+                    rload_across<T>(sm, np, npl, n, nl, base, cj_in,
+                                    [=](int ln, int j1) { return x[j1 * n2 + (c0 + ln)]; });
+                    break;
+                case RT_BLUE_1:
+                    // chirp premultiplied, zero-padded input
+                    rload_across<T>(sm, np, npl, n, nl, base, false, [=](int ln, int j1) {
+                        const int e = j1 * n2 + (c0 + ln);
+                        if (e >= nreal) return cpx<T>{T(0), T(0)};
+                        return dir < 0 ? cmulc(x[e], rd(chirp, e)) : cmul(x[e], rd(chirp, e));
+                    });
+                    break;
+                case RT_FOUR_B:
+                case RT_BLUE_2:
+                    rload_along<T>(are, aim, n, line, tl, tpl, act, cj_in,
+                                   [=](int ln, int j2) { return wsA[(c0 + ln) * n2 + j2]; });
+                    break;
+                case RT_BLUE_3:
+                    rload_across<T>(sm, np, npl, n, nl, base, cj_in,
+                                    [=](int ln, int k1) { return wsA[k1 * n2 + (c0 + ln)]; });
+                    break;
+                default:  // RT_BLUE_S: m points, the first n chirp-premultiplied
+                    rload_along<T>(are, aim, n, line, tl, tpl, act, false, [=](int ln, int k) {
+                        if (k >= nreal) return cpx<T>{T(0), T(0)};
+                        const cpx<T> v = data[off[tl0[ln]] + k];
+                        return dir < 0 ? cmulc(v, rd(chirp, k)) : cmul(v, rd(chirp, k));
+                    });
+                    break;
+            }
+#pragma clang loop unroll(disable)
+            for (int p = 0; p < npass; p++) {
+                if (p == 1 && act) {
+                    // kernel spectrum product (RT_BLUE_2: point k2 of row k1
+                    // is element k1 + n1*k2 of the spectrum), conjugated for
+                    // the inverse pass
+                    for (int k = tl; k < n; k += tpl) {
+                        cpx<T> v = {are[gpos(k)], aim[gpos(k)]};
+                        const cpx<T> bb = rd(bspec, type == RT_BLUE_2 ? (c0 + line) + n1 * k : k);
+                        v = dir < 0 ? cmul(v, bb) : cmulc(v, bb);
+                        are[gpos(k)] = v.x;
+                        aim[gpos(k)] = -v.y;
+                    }
+                }
+                stages_fwd<T>(LL, tw, are, aim, tl, tpl, act);
+            }
+            switch (type) {
+                case RT_SINGLE:
+                    rstore_along<T>(are, aim, n, line, tl, tpl, act, cj_out, [=](int ln, int k, cpx<T> v) {
+                        data[off[tl0[ln]] + k] = cpx<T>{v.x * scale, v.y * scale};
+                    });
+                    break;
+                case RT_FOUR_A:
+                case RT_BLUE_1:
+                    // times exp(d1 2 pi i j2 k1 / N), into the workspace
+                    rstore_across<T>(sm, np, npl, n, nl, base, cj_out, [=](int ln, int k1, cpx<T> v) {
+                        const int j2 = c0 + ln;
+                        wsA[k1 * n2 + j2] = cmul(v, rtw2<T>(d1, tw, B, (int64_t)j2 * k1));
+                    });
+                    break;
+                case RT_FOUR_B:
+                    // output point k2 of row k1 is element k1 + n1*k2
+                    rstore_across<T>(sm, np, npl, n, nl, base, cj_out, [=](int ln, int k2, cpx<T> v) {
+                        x[(c0 + ln) + n1 * k2] = cpx<T>{v.x * scale, v.y * scale};
+                    });
+                    break;
+                case RT_BLUE_2:
+                    // back to j2, times exp(+2 pi i j2 k1 / N), in place
+                    rstore_along<T>(are, aim, n, line, tl, tpl, act, cj_out, [=](int ln, int j2, cpx<T> v) {
+                        const int k1 = c0 + ln;
+                        wsA[k1 * n2 + j2] = cmul(v, rtw2<T>(1, tw, B, (int64_t)j2 * k1));
+                    });
+                    break;
+                case RT_BLUE_3:
+                    // point j1*n2 + j2 of the convolution; chirp postmul of
+                    // the first n points into the transform
+                    rstore_across<T>(sm, np, npl, n, nl, base, cj_out, [=](int ln, int j1, cpx<T> v) {
+                        const int K = j1 * n2 + (c0 + ln);
+                        if (K < nreal) {
+                            const cpx<T> c = rd(chirp, K);
+                            v = dir < 0 ? cmulc(v, c) : cmul(v, c);
+                            x[K] = cpx<T>{v.x * scale, v.y * scale};
+                        }
+                    });
+                    break;
+                default:  // RT_BLUE_S: chirp postmul of the first n points
+                    rstore_along<T>(are, aim, nreal, line, tl, tpl, act, cj_out, [=](int ln, int k, cpx<T> v) {
+                        const cpx<T> c = rd(chirp, k);
+                        v = dir < 0 ? cmulc(v, c) : cmul(v, c);
+                        data[off[tl0[ln]] + k] = cpx<T>{v.x * scale, v.y * scale};
+                    });
+                    break;
+            }
+        }
+        if (type == RT_FOUR_A || type == RT_BLUE_1) rsignal(c3 + 0, cnt);
+        else if (type == RT_BLUE_2) rsignal(c3 + 1, cnt);
     }
 }
 
@@ -742,14 +767,15 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
         for (int c = 0; c < total_lines; c += per) v.push_back(RItem{type, len, c, std::min(per, total_lines - c), tr});
     };
     std::vector<std::pair<double, RItem>> sgl;
-    std::map<int, std::vector<int64_t>> by_len;  // single-kernel transforms grouped by length
+    std::map<int, std::vector<int64_t>> by_len;       // single-kernel transforms grouped by length
+    std::map<int, std::vector<int64_t>> by_len_blue;  // in-LDS Bluestein transforms grouped by length
     std::vector<int64_t> tlist;
     for (int64_t i = 0; i < count; i++) {
         const RLen& L = lens[tlen[i]];
         if (L.kind == RK_SINGLE) {
             by_len[tlen[i]].push_back(i);
         } else if (L.kind == RK_BLUE_S) {
-            sgl.push_back({3.0 * L.m * std::log2((double)L.m), RItem{RT_BLUE_S, tlen[i], 0, 1, i}});
+            by_len_blue[tlen[i]].push_back(i);
         } else if (L.kind == RK_FOUR) {
             split_items(ph[0], RT_FOUR_A, tlen[i], i, L.n2, L.n1);
             split_items(ph[1], RT_FOUR_B, tlen[i], i, L.n1, L.n2);
@@ -760,17 +786,25 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
             split_items(ph[2], RT_BLUE_3, tlen[i], i, M.n2, M.n1);
         }
     }
-    for (auto& kv : by_len) {
-        const RLen& L = lens[kv.first];
-        const int per = std::max(1, (int)(CHUNK / std::max(1, L.n)));
-        const auto& v = kv.second;
-        for (size_t c = 0; c < v.size(); c += per) {
-            int cnt = (int)std::min<size_t>(per, v.size() - c);
-            RItem it{RT_SINGLE, kv.first, (int)tlist.size(), cnt, v[c]};
-            for (int q = 0; q < cnt; q++) tlist.push_back(v[c + q]);
-            sgl.push_back({(double)cnt * L.n * std::log2((double)L.n + 1), it});
+    // whole-transform items: up to CHUNK points of one length (the line
+    // length m for Bluestein), listed from tlist[c0]
+    auto group_items = [&](const std::map<int, std::vector<int64_t>>& groups, int type) {
+        for (auto& kv : groups) {
+            const RLen& L = lens[kv.first];
+            const int ln = type == RT_BLUE_S ? L.m : L.n;
+            const double w = type == RT_BLUE_S ? 3.0 : 1.0;  // two FFT_m passes plus the chirp products
+            const int per = std::max(1, (int)(CHUNK / std::max(1, ln)));
+            const auto& v = kv.second;
+            for (size_t c = 0; c < v.size(); c += per) {
+                int cnt = (int)std::min<size_t>(per, v.size() - c);
+                RItem it{type, kv.first, (int)tlist.size(), cnt, v[c]};
+                for (int q = 0; q < cnt; q++) tlist.push_back(v[c + q]);
+                sgl.push_back({w * cnt * ln * std::log2((double)ln + 1), it});
+            }
         }
-    }
+    };
+    group_items(by_len, RT_SINGLE);
+    group_items(by_len_blue, RT_BLUE_S);
     std::stable_sort(sgl.begin(), sgl.end(), [](auto& a, auto& b) { return a.first > b.first; });
     std::vector<RItem> items;
     const size_t q = (sgl.size() + 2) / 3;

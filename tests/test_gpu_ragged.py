@@ -60,6 +60,34 @@ def test_ragged_mixed(orc, prec, graph):
         assert rel(a[seg], x[seg]) <= 2 * tol(n, prec), n
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_ragged_repeated_lengths(orc, prec):
+    """Transforms of one length share work items (several lines per item):
+    repeated in-LDS lengths, in-LDS Bluestein lengths (17, 211, 2049) and
+    four-step lengths, interleaved, forward and scaled inverse."""
+    import torch
+    lens = np.array([17, 100, 211, 17, 12800, 2049, 100, 17, 211, 12800, 2049, 17, 100, 211, 17, 4096, 4096],
+                    dtype=np.int64)
+    p = jt.RaggedBatch1D(lens, precision=prec)
+    x = orc.jrandom_doubles(20110604, int(2 * lens.sum())).astype(DT[prec])
+    t = torch.from_numpy(x.copy()).cuda()
+    p.complexForward(t)
+    torch.cuda.synchronize()
+    p.sync_status()
+    y = t.cpu().numpy()
+    for i, n in enumerate(lens):
+        seg = slice(int(p.offsets[i]), int(p.offsets[i + 1]))
+        e = rel(y[seg], orc.complex_forward(int(n), x[seg].astype(np.float64)))
+        assert e <= tol(n, prec), (i, n, e)
+    p.complexInverse(t, True)
+    torch.cuda.synchronize()
+    p.sync_status()
+    z = t.cpu().numpy()
+    for i, n in enumerate(lens):
+        seg = slice(int(p.offsets[i]), int(p.offsets[i + 1]))
+        assert rel(z[seg], x[seg]) <= 2 * tol(n, prec), (i, n)
+
+
 def test_ragged_lengths_spec():
     L = ragged_lengths(20000)
     assert L.min() >= 16 and L.max() <= 65536
