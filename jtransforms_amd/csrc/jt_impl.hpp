@@ -269,7 +269,33 @@ std::unique_ptr<RaggedPlan<T>> make_ragged_tuned(const int64_t* lengths, int64_t
                                                  std::string* desc) {
     std::unique_ptr<RaggedPlan<T>> P;
     if (!tune) {
-        P = make_ragged<T>(lengths, count);
+        // JT_RAGGED_SCHEDULE="wgs=2,nt=512,tplx=0,chunk=8,order=0,lag=16" (any
+        // subset; the format of jt_plan_describe) fixes the schedule of
+        // untuned plans, for tests and measurements of one schedule
+        RTune t;
+        if (const char* e = std::getenv("JT_RAGGED_SCHEDULE")) {
+            std::string sch(e);
+            size_t pos = 0;
+            while (pos < sch.size()) {
+                size_t end = sch.find(',', pos);
+                if (end == std::string::npos) end = sch.size();
+                const std::string kv = sch.substr(pos, end - pos);
+                const size_t eq = kv.find('=');
+                if (eq != std::string::npos) {
+                    const std::string k = kv.substr(0, eq);
+                    const int v = std::atoi(kv.c_str() + eq + 1);
+                    if (k == "wgs") t.wgs = v;
+                    else if (k == "nt") t.nt = v;
+                    else if (k == "tplx") t.tplx = v;
+                    else if (k == "chunk") t.chunk = v;
+                    else if (k == "order") t.order = v;
+                    else if (k == "lag") t.lag = v;
+                    else throw JtError(ST_BAD_ARG, "JT_RAGGED_SCHEDULE: unknown key " + k);
+                }
+                pos = end + 1;
+            }
+        }
+        P = make_ragged<T>(lengths, count, t);
     } else {
         auto key = std::make_pair((int)sizeof(T), std::vector<int64_t>(lengths, lengths + count));
         RTune known;

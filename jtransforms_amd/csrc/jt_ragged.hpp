@@ -26,6 +26,7 @@
 #pragma once
 #include <algorithm>
 #include <cstdlib>
+#include <deque>
 #include <functional>
 #include <map>
 
@@ -99,14 +100,21 @@ struct RArgs {
 //   chunk  lines per four-step work item, chunk * (single-kernel limit) points
 //   order  radix order of every runtime plan: 0 = 16, 8/4/2, then odd primes
 //          ascending; 1 = the reverse
+//   lag    queue order of the multi-phase transforms (four-step, Bluestein
+//          over a four-step m).  0: phase by phase (every first phase, then
+//          every second, ...).  > 0: transform by transform, largest first,
+//          each later phase queued once lag * 2^20 points of other work
+//          follow the phase before it, so that it reads the workspace
+//          written shortly before, from the last-level cache, rather than
+//          from HBM, and its wait for that phase has normally ended
 struct RTune {
-    int wgs = 2, nt = 0, tplx = 0, chunk = 4, order = 0;  // nt 0: smallest that fits
+    int wgs = 2, nt = 0, tplx = 0, chunk = 4, order = 0, lag = 0;  // nt 0: smallest that fits
     std::string str() const {
         return "wgs=" + std::to_string(wgs) + ",nt=" + std::to_string(nt) + ",tplx=" + std::to_string(tplx) +
-               ",chunk=" + std::to_string(chunk) + ",order=" + std::to_string(order);
+               ",chunk=" + std::to_string(chunk) + ",order=" + std::to_string(order) + ",lag=" + std::to_string(lag);
     }
     bool operator==(const RTune& o) const {
-        return wgs == o.wgs && nt == o.nt && tplx == o.tplx && chunk == o.chunk && order == o.order;
+        return wgs == o.wgs && nt == o.nt && tplx == o.tplx && chunk == o.chunk && order == o.order && lag == o.lag;
     }
 };
 
@@ -758,8 +766,15 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
     P->total = tot;
     P->lens_of.assign(lengths, lengths + count);
 
-    // work items: cost-balanced phases interleaved with single items
-    std::vector<RItem> ph[3], singles;
+    // work items: the phases of the multi-phase transforms, and whole-transform
+    // items ordered by cost
+    std::vector<RItem> ph[3];
+    struct Multi {
+        std::vector<RItem> ph[3];
+        int nph;
+        int64_t pts;  // points one phase reads or writes (n, or m for Bluestein)
+    };
+    std::vector<Multi> multi;
     // four-step items carry ~4 LDS loads of lines each
     const int CHUNK = (int)(tune.chunk * lim);
     auto split_items = [&](std::vector<RItem>& v, int type, int len, int64_t tr, int total_lines, int line_len) {
@@ -777,13 +792,21 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
         } else if (L.kind == RK_BLUE_S) {
             by_len_blue[tlen[i]].push_back(i);
         } else if (L.kind == RK_FOUR) {
-            split_items(ph[0], RT_FOUR_A, tlen[i], i, L.n2, L.n1);
-            split_items(ph[1], RT_FOUR_B, tlen[i], i, L.n1, L.n2);
+            Multi mt;
+            mt.nph = 2;
+            mt.pts = L.n;
+            split_items(mt.ph[0], RT_FOUR_A, tlen[i], i, L.n2, L.n1);
+            split_items(mt.ph[1], RT_FOUR_B, tlen[i], i, L.n1, L.n2);
+            multi.push_back(std::move(mt));
         } else {
             const RLen& M = lens[L.lm];
-            split_items(ph[0], RT_BLUE_1, tlen[i], i, M.n2, M.n1);
-            split_items(ph[1], RT_BLUE_2, tlen[i], i, M.n1, M.n2);
-            split_items(ph[2], RT_BLUE_3, tlen[i], i, M.n2, M.n1);
+            Multi mt;
+            mt.nph = 3;
+            mt.pts = M.n;
+            split_items(mt.ph[0], RT_BLUE_1, tlen[i], i, M.n2, M.n1);
+            split_items(mt.ph[1], RT_BLUE_2, tlen[i], i, M.n1, M.n2);
+            split_items(mt.ph[2], RT_BLUE_3, tlen[i], i, M.n2, M.n1);
+            multi.push_back(std::move(mt));
         }
     }
     // whole-transform items: up to CHUNK points of one length (the line
@@ -806,11 +829,60 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
     group_items(by_len, RT_SINGLE);
     group_items(by_len_blue, RT_BLUE_S);
     std::stable_sort(sgl.begin(), sgl.end(), [](auto& a, auto& b) { return a.first > b.first; });
+    // Every later phase of a transform sits after its earlier phases in the
+    // queue (the waits of the kernel rely on it).
     std::vector<RItem> items;
-    const size_t q = (sgl.size() + 2) / 3;
-    for (int p = 0; p < 3; p++) {
-        items.insert(items.end(), ph[p].begin(), ph[p].end());
-        for (size_t s = p * q; s < std::min(sgl.size(), (p + 1) * q); s++) items.push_back(sgl[s].second);
+    if (tune.lag <= 0) {
+        // phase by phase, a third of the whole-transform items after each
+        for (auto& mt : multi)
+            for (int p = 0; p < mt.nph; p++) ph[p].insert(ph[p].end(), mt.ph[p].begin(), mt.ph[p].end());
+        const size_t q = (sgl.size() + 2) / 3;
+        for (int p = 0; p < 3; p++) {
+            items.insert(items.end(), ph[p].begin(), ph[p].end());
+            for (size_t s = p * q; s < std::min(sgl.size(), (p + 1) * q); s++) items.push_back(sgl[s].second);
+        }
+    } else {
+        const int64_t LAG = (int64_t)tune.lag << 20;
+        std::vector<size_t> ord(multi.size());
+        for (size_t k = 0; k < ord.size(); k++) ord[k] = k;
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return multi[a].pts > multi[b].pts; });
+        struct Pend {
+            int64_t mark;  // points queued when the phase before it was queued
+            size_t idx;
+            int phase;
+        };
+        std::deque<Pend> pend;
+        int64_t queued = 0;
+        size_t next_sgl = 0;
+        auto emit = [&](size_t idx, int p) {
+            const Multi& mt = multi[idx];
+            items.insert(items.end(), mt.ph[p].begin(), mt.ph[p].end());
+            queued += mt.pts;
+            if (p + 1 < mt.nph) pend.push_back(Pend{queued, idx, p + 1});
+        };
+        auto ready = [&]() {
+            while (!pend.empty() && queued - pend.front().mark >= LAG) {
+                const Pend q = pend.front();
+                pend.pop_front();
+                emit(q.idx, q.phase);
+            }
+        };
+        for (size_t k = 0; k < ord.size(); k++) {
+            emit(ord[k], 0);
+            ready();
+        }
+        // the remaining later phases, with whole-transform items as filler
+        while (!pend.empty()) {
+            while (next_sgl < sgl.size() && queued - pend.front().mark < LAG) {
+                const RItem& it = sgl[next_sgl++].second;
+                items.push_back(it);
+                queued += (int64_t)it.cnt * (it.type == RT_BLUE_S ? lens[it.len].m : lens[it.len].n);
+            }
+            const Pend q = pend.front();
+            pend.pop_front();
+            emit(q.idx, q.phase);
+        }
+        for (; next_sgl < sgl.size(); next_sgl++) items.push_back(sgl[next_sgl].second);
     }
     P->nitems = (int)items.size();
 
@@ -1023,14 +1095,15 @@ std::unique_ptr<RaggedPlan<T>> ragged_autotune(const int64_t* lengths, int64_t c
                 t.nt = nt;
                 if (w * nt <= 2048) consider(t);
             }
-        for (int knob = 0; knob < 3; knob++) {
+        for (int knob = 0; knob < 4; knob++) {
             const std::vector<int> vals = knob == 0 ? std::vector<int>{0, 1, 2}
                                         : knob == 1 ? std::vector<int>{2, 4, 8, 16, 32}
-                                                    : std::vector<int>{0, 1};
+                                        : knob == 2 ? std::vector<int>{0, 1}
+                                                    : std::vector<int>{0, 2, 8, 32, 128};
             const RTune base = best->tune;
             for (int v : vals) {
                 RTune t = base;
-                (knob == 0 ? t.tplx : knob == 1 ? t.chunk : t.order) = v;
+                (knob == 0 ? t.tplx : knob == 1 ? t.chunk : knob == 2 ? t.order : t.lag) = v;
                 consider(t);
             }
         }

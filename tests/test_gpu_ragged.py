@@ -88,6 +88,29 @@ def test_ragged_repeated_lengths(orc, prec):
         assert rel(z[seg], x[seg]) <= 2 * tol(n, prec), (i, n)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("sched", ["lag=0", "lag=1,chunk=2", "lag=16,chunk=8", "wgs=1,nt=1024,lag=2",
+                                   "order=1,tplx=1,lag=4"])
+def test_ragged_schedules(orc, prec, sched, monkeypatch):
+    """Explicit schedules (queue order by phase or transform by transform,
+    item size, workgroup shape, radix order) give the same transforms."""
+    import torch
+    monkeypatch.setenv("JT_RAGGED_SCHEDULE", sched)
+    lens = np.array(MIXED + [12800, 65536, 10007, 6561, 100, 17, 2049], dtype=np.int64)
+    p = jt.RaggedBatch1D(lens, precision=prec)
+    assert all(kv in p.describe() for kv in sched.split(",")), p.describe()
+    x = orc.jrandom_doubles(20110605, int(2 * lens.sum())).astype(DT[prec])
+    t = torch.from_numpy(x.copy()).cuda()
+    p.complexForward(t)
+    torch.cuda.synchronize()
+    p.sync_status()
+    y = t.cpu().numpy()
+    for i, n in enumerate(lens):
+        seg = slice(int(p.offsets[i]), int(p.offsets[i + 1]))
+        e = rel(y[seg], orc.complex_forward(int(n), x[seg].astype(np.float64)))
+        assert e <= tol(n, prec), (i, n, e)
+
+
 def test_ragged_lengths_spec():
     L = ragged_lengths(20000)
     assert L.min() >= 16 and L.max() <= 65536
