@@ -193,17 +193,20 @@ def c3_forms(torch, jt, steps, warmup):
     res = {}
     for form in ("plain", "split_fp16"):
         plan = jt.FFTBatch1D(n, batch, precision="f32", split_fp16=(form == "split_fp16"))
-        # step time without per-launch events, kernel durations from a second
-        # timed run with them
-        bufs = [base.clone() for _ in range(warmup + steps)]
-        ms, _ = timed(torch, jt, [(lambda b=b: plan.complexForward(b)) for b in bufs], steps, warmup,
-                      profile=False)
-        del bufs
+        # kernel durations (and the accuracy check) from a timed run with
+        # per-launch events; the step time from a second run without them.
+        # The first run over freshly allocated buffers also serves as the
+        # warm-up of the second: on a fresh box the first passes over new
+        # allocations run up to 10% slower
         bufs = [base.clone() for _ in range(warmup + steps)]
         ms_ev, prof = timed(torch, jt, [(lambda b=b: plan.complexForward(b)) for b in bufs], steps, warmup)
         y = bufs[-1].view(batch, 2 * n)[lines].cpu().numpy().astype(np.float64)
         got = y[:, 0::2] + 1j * y[:, 1::2]
         err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        del bufs
+        bufs = [base.clone() for _ in range(warmup + steps)]
+        ms, _ = timed(torch, jt, [(lambda b=b: plan.complexForward(b)) for b in bufs], steps, warmup,
+                      profile=False)
         rf = roofline_from_profile(prof)
         rf.pop("kernels")
         res[form] = {"ms_per_step": ms, "GFLOPs": flops / (ms * 1e-3) / 1e9,
