@@ -157,6 +157,22 @@ def roofline_from_profile(prof, alg_override=None):
             "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": per, "kernels": kernels}
 
 
+def prewarm(torch, calls):
+    """One untimed call per buffer: on a fresh box the first passes over new
+    device allocations run up to 10% slower, which the few warm-up steps of a
+    timed run do not cover (they touch only their own buffers)."""
+    for c in calls:
+        c()
+    torch.cuda.synchronize()
+
+
+def refill(torch, bufs, base):
+    """Restore every input buffer to the same synthetic input (in place)."""
+    for b in bufs:
+        b.copy_(base)
+    torch.cuda.synchronize()
+
+
 def timed(torch, jt, calls, steps, warmup, profile=True):
     """Run calls[i]() for warm-up then timed steps; returns (ms/step, profile).
     With profile=False the library records no per-launch events (None is
@@ -193,20 +209,18 @@ def c3_forms(torch, jt, steps, warmup):
     res = {}
     for form in ("plain", "split_fp16"):
         plan = jt.FFTBatch1D(n, batch, precision="f32", split_fp16=(form == "split_fp16"))
-        # kernel durations (and the accuracy check) from a timed run with
-        # per-launch events; the step time from a second run without them.
-        # The first run over freshly allocated buffers also serves as the
-        # warm-up of the second: on a fresh box the first passes over new
-        # allocations run up to 10% slower
         bufs = [base.clone() for _ in range(warmup + steps)]
-        ms_ev, prof = timed(torch, jt, [(lambda b=b: plan.complexForward(b)) for b in bufs], steps, warmup)
+        prewarm(torch, [(lambda b=b: plan.complexForward(b)) for b in bufs])
+        calls = [(lambda b=b: plan.complexForward(b)) for b in bufs]
+        # the step time without per-launch events; kernel durations (and the
+        # accuracy check) from the same steps run again with them
+        refill(torch, bufs, base)
+        ms, _ = timed(torch, jt, calls, steps, warmup, profile=False)
+        refill(torch, bufs, base)
+        ms_ev, prof = timed(torch, jt, calls, steps, warmup)
         y = bufs[-1].view(batch, 2 * n)[lines].cpu().numpy().astype(np.float64)
         got = y[:, 0::2] + 1j * y[:, 1::2]
         err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-        del bufs
-        bufs = [base.clone() for _ in range(warmup + steps)]
-        ms, _ = timed(torch, jt, [(lambda b=b: plan.complexForward(b)) for b in bufs], steps, warmup,
-                      profile=False)
         rf = roofline_from_profile(prof)
         rf.pop("kernels")
         res[form] = {"ms_per_step": ms, "GFLOPs": flops / (ms * 1e-3) / 1e9,
@@ -252,6 +266,8 @@ def other_workload(args, torch, jt):
                 plan.set_split_fp16(True)
             bufs = [base.clone() for _ in range(W + K)]
             calls = [(lambda b=b: plan.complexForward(b)) for b in bufs]
+            prewarm(torch, calls)
+            refill(torch, bufs, base)
             ms, prof = timed(torch, jt, calls, K, W)
             res[form] = {"ms_per_step": ms, "GFLOPs": flops / (ms * 1e-3) / 1e9,
                          "rel_l2_vs_fp64_gpu": rel(bufs[-1], ref), "roofline": roofline_from_profile(prof)}
@@ -268,6 +284,8 @@ def other_workload(args, torch, jt):
         base = torch.rand(n_pts, dtype=torch.float64, device=dev, generator=g) * 2 - 1
         bufs = [base.clone() for _ in range(W + K)]
         calls = [(lambda b=b: plan.forward(b, True)) for b in bufs]
+        prewarm(torch, calls)
+        refill(torch, bufs, base)
         ms, prof = timed(torch, jt, calls, K, W)
         chk = base.clone()
         plan.forward(chk, True)
@@ -496,11 +514,14 @@ def main():
     base_r = torch.rand(ROWS * COLS, dtype=torch.float64, device=dev, generator=g) * 2 - 1
     total = args.warmup + args.steps
 
+    bufs = [(base_c.clone(), base_r.clone()) for _ in range(total)]
+    prewarm(torch, [(lambda b=b: (plan.complexForward(b[0]), plan.realForward(b[1]))) for b in bufs])
+
     def run_steps(instrumented):
         """W warm-up steps, then K timed steps on fresh input copies; with
         `instrumented` the library records HIP events around every launch."""
-        bufs = [(base_c.clone(), base_r.clone()) for _ in range(total)]
-        torch.cuda.synchronize()
+        refill(torch, [b[0] for b in bufs], base_c)
+        refill(torch, [b[1] for b in bufs], base_r)
         for i in range(args.warmup):
             plan.complexForward(bufs[i][0])
             plan.realForward(bufs[i][1])
@@ -515,7 +536,6 @@ def main():
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         prof = jt.profile_end() if instrumented else None
-        del bufs
         return el * 1e3 / args.steps, prof
 
     # the step time is measured without per-launch events; the kernel
@@ -523,6 +543,7 @@ def main():
     # the events recorded (their own step time is reported beside it)
     ms_per_step, _ = run_steps(False)
     ms_profiled, prof = run_steps(True)
+    del bufs
     value = flops_per_step() / (ms_per_step * 1e-3) / 1e9
 
     # dominant kernel: the instantiation (launch label without its line
