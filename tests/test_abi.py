@@ -61,3 +61,21 @@ def test_null_plan_rejected():
     assert L.jt_execute(None, 0, None, 0, 0, None) == jt.JT_BAD_ARG
     assert L.jt_execute_host(None, 0, None, 0, 0) == jt.JT_BAD_ARG
     assert L.jt_plan_elements(None, 0) == -1
+
+
+@pytest.mark.parametrize("lens,msg", [([], "count must be greater than 0"),
+                                      ([0], "n must be greater than 0"),
+                                      ([5, -1], "n must be greater than 0")])
+def test_ragged_illegal_lengths(lens, msg):
+    """Ragged batches take the reference's 1D size rule for every length
+    (DoubleFFT_1D.java:119-121) and need at least one transform."""
+    import numpy as np
+    with pytest.raises(jt.IllegalArgumentException, match=re.escape(msg)):
+        jt.RaggedBatch1D(np.array(lens, dtype=np.int64))
+
+
+def test_ragged_schedule_override_rejects_unknown_key(monkeypatch):
+    import numpy as np
+    monkeypatch.setenv("JT_RAGGED_SCHEDULE", "wgs=2,bogus=1")
+    with pytest.raises(ValueError, match="unknown key bogus"):
+        jt.RaggedBatch1D(np.array([16, 32], dtype=np.int64))
