@@ -633,6 +633,11 @@ std::unique_ptr<RaggedPlan<T>> make_ragged(const int64_t* lengths, int64_t count
     P->tune = tune;
     // longest line kept in one workgroup's LDS (re and im arrays)
     const int64_t lim = sizeof(T) == 8 ? 4096 : 8192;
+    if (tune.wgs < 1 || tune.wgs > 8 || !(tune.nt == 0 || tune.nt == 256 || tune.nt == 512 || tune.nt == 1024) ||
+        tune.tplx < 0 || tune.tplx > 4 || tune.chunk < 1 || tune.chunk > 1024 || tune.order < 0 || tune.order > 1 ||
+        tune.lag < 0 || tune.lag > (1 << 20))
+        throw JtError(ST_BAD_ARG, "ragged schedule out of range (wgs 1..8, nt 0/256/512/1024, tplx 0..4, "
+                                  "chunk 1..1024, order 0/1, lag 0..2^20): " + tune.str());
     std::vector<RLen> lens;
     std::vector<cpx<T>> tab;
     int max_tpl = 1;                     // largest threads-per-line requirement

@@ -79,3 +79,12 @@ def test_ragged_schedule_override_rejects_unknown_key(monkeypatch):
     monkeypatch.setenv("JT_RAGGED_SCHEDULE", "wgs=2,bogus=1")
     with pytest.raises(ValueError, match="unknown key bogus"):
         jt.RaggedBatch1D(np.array([16, 32], dtype=np.int64))
+
+
+@pytest.mark.parametrize("sched", ["tplx=40", "wgs=0", "nt=300", "chunk=0", "order=2", "lag=-1"])
+def test_ragged_schedule_override_range_checked(monkeypatch, sched):
+    """Out-of-range schedule values are refused before any launch."""
+    import numpy as np
+    monkeypatch.setenv("JT_RAGGED_SCHEDULE", sched)
+    with pytest.raises(ValueError, match="ragged schedule out of range"):
+        jt.RaggedBatch1D(np.array([16, 32], dtype=np.int64))
