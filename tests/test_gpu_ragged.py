@@ -86,6 +86,15 @@ def test_ragged_repeated_lengths(orc, prec):
     for i, n in enumerate(lens):
         seg = slice(int(p.offsets[i]), int(p.offsets[i + 1]))
         assert rel(z[seg], x[seg]) <= 2 * tol(n, prec), (i, n)
+    # unscaled inverse of the forward result: n times the input
+    t.copy_(torch.from_numpy(y))
+    p.complexInverse(t, False)
+    torch.cuda.synchronize()
+    p.sync_status()
+    w = t.cpu().numpy()
+    for i, n in enumerate(lens):
+        seg = slice(int(p.offsets[i]), int(p.offsets[i + 1]))
+        assert rel(w[seg], float(n) * x[seg].astype(np.float64)) <= 2 * tol(n, prec), (i, n)
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
